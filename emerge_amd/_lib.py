@@ -1,0 +1,70 @@
+"""ctypes binding of libemerge_amd.so (the C ABI declared in include/emerge_amd.h).
+
+The library is required: every GPU entry point raises if it is missing, there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemerge_amd.so")
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int32
+F64 = ctypes.c_double
+F32 = ctypes.c_float
+
+# name -> (restype, argtypes).  Keep in step with include/emerge_amd.h.
+SIGNATURES: dict[str, tuple] = {
+    "emg_last_error": (ctypes.c_char_p, []),
+    "emg_pattern_build": (ctypes.c_int, [P, I64, I64, P, ctypes.POINTER(P)]),
+    "emg_pattern_info": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+    "emg_pattern_csr": (ctypes.c_int, [P, P, P, P]),
+    "emg_pattern_destroy": (None, [P]),
+    "emg_assemble_iso": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, P, P, P]),
+    "emg_assemble_tensor": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, P, P, P]),
+}
+
+_lib = None
+
+
+class EmgError(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Load libemerge_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EmgError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                           "or `python emerge_amd/build.py`; there is no CPU fallback")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().emg_last_error().decode(errors="replace")
+        raise EmgError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t) -> ctypes.c_void_p:
+    """Device (or host) pointer of a torch tensor / numpy array."""
+    if hasattr(t, "data_ptr"):
+        return ctypes.c_void_p(t.data_ptr())
+    return t.ctypes.data_as(ctypes.c_void_p)
+
+
+def stream_ptr(stream=None) -> ctypes.c_void_p:
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)

@@ -1,0 +1,507 @@
+// Nedelec-2 tetrahedral assembly on MI355X (gfx950).
+//
+// Replaces the reference's element loop and COO->CSR conversion
+// (fem/physics/edm/optimized_assembly.py:43-116 with fem/mth/tet.py:854-1088).
+//
+// Design
+//  * Entities = [edges | faces].  DOF = mode*n_ent + entity, which is the
+//    reference's [e1 | f1 | e2 | f2] numbering (fem/elements/nedelec2.py:46-62),
+//    so the global matrix is a 2x2 block matrix whose blocks share the
+//    entity-adjacency pattern.  The symbolic step builds that pattern once per
+//    mesh with a GPU radix sort of the 100 (entity, entity) keys of every tet.
+//  * The sorted values of that sort form, for every adjacency entry (e, e'),
+//    the list of tets containing both entities with their local slots.
+//  * Numeric assembly is a row-owner gather: one thread per (e, e') computes
+//    the 2x2 (mode, mode') sub-blocks of the curl-curl (E) and mass (B)
+//    integrals for each shared tet and writes each CSR value exactly once.
+//    No atomics, no zero fill, results are deterministic.
+//  * Tets are canonical (vertices sorted by global id), which is exactly the
+//    orientation the reference derives from its global-id local maps
+//    (fem/mth/optimized.py:273-307), so all local tables are constants.
+//
+// Element integrals, per basis function N = s*(l_p l_q grad l_r - l_p l_r grad l_q):
+//   mass   F_ij = s_i s_j /(6V*5040) * sum of 4 terms c4(..) g_x.Mm g_y
+//   curl   curl N = s/(6V)^2 * sum_t cf_t l_{v_t} K_t  (K_t = g_x x g_y)
+//          D_ij = s_i s_j /(120 (6V)^3) * sum_{t,t'} cf cf' (1+[v_t==v_t']) K_t.Ms K'_t'
+// where g_k = 6V grad(lambda_k) and c4 = prod(multiplicity!) is the
+// barycentric-monomial moment numerator (fem/mth/optimized.py:251-260).
+
+#include <cstdio>
+#include <cstring>
+#include <thread>
+
+#include <rocprim/rocprim.hpp>
+
+#include "../../include/emerge_amd.h"
+#include "emg_common.h"
+
+using emg::cplx;
+
+static void hfree(void *p) { (void)hipFree(p); }
+
+namespace emg {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace emg
+
+extern "C" const char *emg_last_error(void) { return emg::g_err.c_str(); }
+
+struct emg_pattern {
+    int64_t nt = 0, n_ent = 0, nnz_ent = 0, n_inc = 0, nblk = 0;
+    int bits = 0;
+    int32_t *ent_ptr = nullptr;   // n_ent + 1
+    int32_t *adj = nullptr;       // nnz_ent, column entity of each adjacency entry
+    int32_t *pair_ptr = nullptr;  // nnz_ent + 1, into inc
+    uint32_t *inc = nullptr;      // n_inc = 100*nt : (tet << 8) | (s << 4) | s'
+    int32_t *blk_row0 = nullptr;  // nblk + 1, first row of each 256-entry block
+};
+
+static constexpr int PAIR_BLOCK = 256;
+
+// ------------------------------------------------------------------ symbolic
+__global__ void k_make_keys(const int32_t *__restrict__ tet_ent, int64_t nt, int bits,
+                            uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int32_t e[10];
+#pragma unroll
+    for (int s = 0; s < 10; s++) e[s] = tet_ent[t * 10 + s];
+    int64_t base = t * 100;
+#pragma unroll
+    for (int s = 0; s < 10; s++)
+#pragma unroll
+        for (int s2 = 0; s2 < 10; s2++) {
+            keys[base + s * 10 + s2] = ((uint64_t)(uint32_t)e[s] << bits) | (uint64_t)(uint32_t)e[s2];
+            vals[base + s * 10 + s2] = ((uint32_t)t << 8) | (uint32_t)(s << 4) | (uint32_t)s2;
+        }
+}
+
+__global__ void k_rows_from_keys(const uint64_t *__restrict__ uniq, int64_t nnz_ent, int bits, int64_t n_ent,
+                                 int32_t *__restrict__ ent_ptr, int32_t *__restrict__ adj) {
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k > nnz_ent) return;
+    if (k == nnz_ent) { ent_ptr[n_ent] = (int32_t)nnz_ent; return; }
+    uint64_t key = uniq[k];
+    int64_t row = (int64_t)(key >> bits);
+    adj[k] = (int32_t)(key & ((1ull << bits) - 1));
+    if (k == 0 || (int64_t)(uniq[k - 1] >> bits) != row) ent_ptr[row] = (int32_t)k;
+}
+
+__global__ void k_block_rows(const uint64_t *__restrict__ uniq, int64_t nnz_ent, int bits, int64_t nblk,
+                             int64_t n_ent, int32_t *__restrict__ blk_row0) {
+    int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b > nblk) return;
+    blk_row0[b] = (b == nblk) ? (int32_t)(n_ent - 1) : (int32_t)(uniq[b * PAIR_BLOCK] >> bits);
+}
+
+__global__ void k_set_last(int32_t *p, int64_t idx, int32_t v) { p[idx] = v; }
+
+// Row entity of adjacency entry k, searched inside the range of its block.
+__device__ __forceinline__ int find_row(const int32_t *__restrict__ ent_ptr, const int32_t *__restrict__ blk_row0,
+                                        int64_t k) {
+    int blk = (int)(k / PAIR_BLOCK);
+    int lo = blk_row0[blk], hi = blk_row0[blk + 1];
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (ent_ptr[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_csr_cols(const emg_pattern pat, int32_t *__restrict__ cols) {
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= pat.nnz_ent) return;
+    int e = find_row(pat.ent_ptr, pat.blk_row0, k);
+    int32_t p0 = pat.ent_ptr[e];
+    int32_t deg = pat.ent_ptr[e + 1] - p0;
+    int32_t c = pat.adj[k];
+    for (int m = 0; m < 2; m++) {
+        int64_t base = (int64_t)m * 2 * pat.nnz_ent + p0 + k;  // = rowptr(m,e) + (k - p0)
+        cols[base] = c;
+        cols[base + deg] = c + (int32_t)pat.n_ent;
+    }
+}
+
+__global__ void k_csr_rowptr(const emg_pattern pat, int32_t *__restrict__ rowptr) {
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t n = 2 * pat.n_ent;
+    if (r > n) return;
+    if (r == n) { rowptr[r] = (int32_t)(4 * pat.nnz_ent); return; }
+    int64_t m = r / pat.n_ent, e = r - m * pat.n_ent;
+    rowptr[r] = (int32_t)(m * 2 * pat.nnz_ent + 2 * (int64_t)pat.ent_ptr[e]);
+}
+
+static int bits_for(int64_t n) {
+    int b = 1;
+    while ((1ll << b) < n) b++;
+    return b;
+}
+
+extern "C" int emg_pattern_build(const int32_t *d_tet_ent, int64_t nt, int64_t n_ent, void *stream, emg_pattern **out) {
+    EMG_CHECK(out != nullptr && d_tet_ent != nullptr, "emg_pattern_build: null argument");
+    EMG_CHECK(nt > 0 && n_ent > 0, "emg_pattern_build: empty mesh");
+    EMG_CHECK(nt < (1ll << 24), "emg_pattern_build: more than 2^24 tets is not supported");
+    hipStream_t st = (hipStream_t)stream;
+    int bits = bits_for(n_ent);
+    EMG_CHECK(2 * bits <= 64 && n_ent < (1ll << 30), "emg_pattern_build: too many entities");
+    const int64_t n_inc = 100 * nt;
+    uint64_t *keys = nullptr, *keys_s = nullptr, *uniq = nullptr;
+    uint32_t *vals = nullptr, *counts = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0, need = 0;
+    int64_t *d_nruns = nullptr;
+    auto pat = new emg_pattern();
+    pat->nt = nt;
+    pat->n_ent = n_ent;
+    pat->n_inc = n_inc;
+    pat->bits = bits;
+    int rc = 0;
+    auto fail = [&](int code) {
+        hfree(keys); hfree(keys_s); hfree(vals); hfree(uniq); hfree(counts); hfree(tmp); hfree(d_nruns);
+        emg_pattern_destroy(pat);
+        return code;
+    };
+#define TRY(call)                                                                 \
+    do {                                                                          \
+        hipError_t _e = (call);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            emg::set_error(std::string(#call) + ": " + hipGetErrorString(_e));    \
+            return fail(-2);                                                      \
+        }                                                                         \
+    } while (0)
+    TRY(hipMalloc(&keys, n_inc * sizeof(uint64_t)));
+    TRY(hipMalloc(&keys_s, n_inc * sizeof(uint64_t)));
+    TRY(hipMalloc(&vals, n_inc * sizeof(uint32_t)));
+    TRY(hipMalloc(&pat->inc, n_inc * sizeof(uint32_t)));
+    k_make_keys<<<emg::grid_for(nt, 256), 256, 0, st>>>(d_tet_ent, nt, bits, keys, vals);
+    TRY(hipGetLastError());
+    TRY(rocprim::radix_sort_pairs(nullptr, need, keys, keys_s, vals, pat->inc, (size_t)n_inc, 0, 2 * bits, st));
+    tmp_bytes = need;
+    TRY(rocprim::run_length_encode(nullptr, need, keys_s, (size_t)n_inc, keys, vals, d_nruns, st));
+    if (need > tmp_bytes) tmp_bytes = need;
+    TRY(hipMalloc(&tmp, tmp_bytes));
+    TRY(hipMalloc(&d_nruns, sizeof(int64_t)));
+    need = tmp_bytes;
+    TRY(rocprim::radix_sort_pairs(tmp, need, keys, keys_s, vals, pat->inc, (size_t)n_inc, 0, 2 * bits, st));
+    // Unique keys go to `keys` (now free), run lengths to `vals`.
+    need = tmp_bytes;
+    TRY(rocprim::run_length_encode(tmp, need, keys_s, (size_t)n_inc, keys, vals, d_nruns, st));
+    int64_t nnz_ent = 0;
+    TRY(hipMemcpyAsync(&nnz_ent, d_nruns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    TRY(hipStreamSynchronize(st));
+    EMG_CHECK(4 * nnz_ent < (1ll << 31) || (fail(-1), false), "emg_pattern_build: nnz exceeds int32 CSR indices");
+    pat->nnz_ent = nnz_ent;
+    uniq = keys;
+    keys = nullptr;
+    counts = vals;
+    vals = nullptr;
+    TRY(hipMalloc(&pat->pair_ptr, (nnz_ent + 1) * sizeof(int32_t)));
+    TRY(hipMalloc(&pat->ent_ptr, (n_ent + 1) * sizeof(int32_t)));
+    TRY(hipMalloc(&pat->adj, nnz_ent * sizeof(int32_t)));
+    pat->nblk = (nnz_ent + PAIR_BLOCK - 1) / PAIR_BLOCK;
+    TRY(hipMalloc(&pat->blk_row0, (pat->nblk + 1) * sizeof(int32_t)));
+    {
+        size_t scan_need = 0;
+        TRY(rocprim::exclusive_scan(nullptr, scan_need, counts, pat->pair_ptr, (int32_t)0, (size_t)nnz_ent,
+                                    rocprim::plus<int32_t>(), st));
+        if (scan_need > tmp_bytes) {
+            hfree(tmp);
+            tmp = nullptr;
+            TRY(hipMalloc(&tmp, scan_need));
+            tmp_bytes = scan_need;
+        }
+        scan_need = tmp_bytes;
+        TRY(rocprim::exclusive_scan(tmp, scan_need, counts, pat->pair_ptr, (int32_t)0, (size_t)nnz_ent,
+                                    rocprim::plus<int32_t>(), st));
+    }
+    k_set_last<<<1, 1, 0, st>>>(pat->pair_ptr, nnz_ent, (int32_t)n_inc);
+    k_rows_from_keys<<<emg::grid_for(nnz_ent + 1, 256), 256, 0, st>>>(uniq, nnz_ent, bits, n_ent, pat->ent_ptr, pat->adj);
+    k_block_rows<<<emg::grid_for(pat->nblk + 1, 256), 256, 0, st>>>(uniq, nnz_ent, bits, pat->nblk, n_ent, pat->blk_row0);
+    TRY(hipGetLastError());
+    TRY(hipStreamSynchronize(st));
+    hfree(uniq);
+    hfree(counts);
+    hfree(keys_s);
+    hfree(tmp);
+    hfree(d_nruns);
+#undef TRY
+    *out = pat;
+    return rc;
+}
+
+extern "C" int emg_pattern_info(const emg_pattern *pat, int64_t *n_rows, int64_t *nnz, int64_t *nnz_ent, int64_t *n_inc) {
+    EMG_CHECK(pat != nullptr, "emg_pattern_info: null pattern");
+    if (n_rows) *n_rows = 2 * pat->n_ent;
+    if (nnz) *nnz = 4 * pat->nnz_ent;
+    if (nnz_ent) *nnz_ent = pat->nnz_ent;
+    if (n_inc) *n_inc = pat->n_inc;
+    return 0;
+}
+
+extern "C" int emg_pattern_csr(const emg_pattern *pat, int32_t *d_rowptr, int32_t *d_cols, void *stream) {
+    EMG_CHECK(pat != nullptr && d_rowptr != nullptr && d_cols != nullptr, "emg_pattern_csr: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    k_csr_rowptr<<<emg::grid_for(2 * pat->n_ent + 1, 256), 256, 0, st>>>(*pat, d_rowptr);
+    k_csr_cols<<<(unsigned)pat->nblk, PAIR_BLOCK, 0, st>>>(*pat, d_cols);
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" void emg_pattern_destroy(emg_pattern *pat) {
+    if (!pat) return;
+    hfree(pat->ent_ptr);
+    hfree(pat->adj);
+    hfree(pat->pair_ptr);
+    hfree(pat->inc);
+    hfree(pat->blk_row0);
+    delete pat;
+}
+
+// ------------------------------------------------------------------ numeric
+// Local slot s (0..9) -> canonical vertices (a, b, c); edges repeat b as c.
+// Edges (0,1)(0,2)(0,3)(1,2)(1,3)(2,3); faces (0,1,2)(0,2,3)(0,1,3)(1,2,3):
+// the reference's local pairs/triples (fem/mesh3d.py:292,296) on sorted tets.
+__device__ __forceinline__ void slot_vertices(int s, int &a, int &b, int &c) {
+    constexpr uint64_t P =
+        (0ull | 1ull << 2 | 1ull << 4) << 0 | (0ull | 2ull << 2 | 2ull << 4) << 6 | (0ull | 3ull << 2 | 3ull << 4) << 12 |
+        (1ull | 2ull << 2 | 2ull << 4) << 18 | (1ull | 3ull << 2 | 3ull << 4) << 24 | (2ull | 3ull << 2 | 3ull << 4) << 30 |
+        (0ull | 1ull << 2 | 2ull << 4) << 36 | (0ull | 2ull << 2 | 3ull << 4) << 42 | (0ull | 1ull << 2 | 3ull << 4) << 48 |
+        (1ull | 2ull << 2 | 3ull << 4) << 54;
+    uint32_t v = (uint32_t)(P >> (6 * s)) & 63u;
+    a = v & 3;
+    b = (v >> 2) & 3;
+    c = (v >> 4) & 3;
+}
+
+// prod(multiplicity!) of the multiset {w, x, y, z} of local vertex ids.
+__device__ __forceinline__ double c4(int w, int x, int y, int z) {
+    uint32_t cnt = (1u << (4 * w)) + (1u << (4 * x)) + (1u << (4 * y)) + (1u << (4 * z));
+    double r = 1.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t n = (cnt >> (4 * k)) & 15u;
+        r *= (n == 4u) ? 24.0 : (n == 3u) ? 6.0 : (n == 2u) ? 2.0 : 1.0;
+    }
+    return r;
+}
+
+template <bool C>
+struct Num;
+template <>
+struct Num<false> {
+    using T = double;
+};
+template <>
+struct Num<true> {
+    using T = cplx;
+};
+
+__device__ __forceinline__ double dmul(double a, double b) { return a * b; }
+__device__ __forceinline__ cplx dmul(cplx a, double b) { return a * b; }
+
+struct V3 {
+    double x, y, z;
+};
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 crs(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double nrm(V3 a) { return sqrt(dot(a, a)); }
+
+// a^T M b for a complex 3x3 tensor M (row-major, interleaved re/im).
+__device__ __forceinline__ cplx qform(V3 a, const cplx *M, V3 b) {
+    cplx r;
+    double av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r += (av[i] * bv[j]) * M[i * 3 + j];
+    return r;
+}
+
+constexpr int ABLK = 128;
+
+// Per-thread staging of the 4 vertices and 4 cofactor gradients in LDS, laid
+// out component-major so dynamic vertex indices stay bank-conflict free.
+struct Stage {
+    double *s;
+    int tid;
+    __device__ __forceinline__ void put(int slot, V3 v) {
+        s[(slot * 3 + 0) * ABLK + tid] = v.x;
+        s[(slot * 3 + 1) * ABLK + tid] = v.y;
+        s[(slot * 3 + 2) * ABLK + tid] = v.z;
+    }
+    __device__ __forceinline__ V3 get(int slot) const {
+        return {s[(slot * 3 + 0) * ABLK + tid], s[(slot * 3 + 1) * ABLK + tid], s[(slot * 3 + 2) * ABLK + tid]};
+    }
+};
+
+template <bool TENSOR>
+__global__ void __launch_bounds__(ABLK) k_assemble(const emg_pattern pat, const double *__restrict__ nodes,
+                                                   const int32_t *__restrict__ tets, const cplx *__restrict__ mat_m,
+                                                   const cplx *__restrict__ mat_s, int exact_face_mass,
+                                                   cplx *__restrict__ E, cplx *__restrict__ B) {
+    using T = typename Num<TENSOR>::T;
+    __shared__ double lds[24 * ABLK];
+    Stage st{lds, (int)threadIdx.x};
+    const int64_t k = blockIdx.x * (int64_t)ABLK + threadIdx.x;
+    if (k >= pat.nnz_ent) return;
+    const int e = find_row(pat.ent_ptr, pat.blk_row0, k);
+    const int32_t p0 = pat.ent_ptr[e];
+    const int32_t deg = pat.ent_ptr[e + 1] - p0;
+    cplx accD[2][2], accF[2][2];
+    const int32_t q0 = pat.pair_ptr[k], q1 = pat.pair_ptr[k + 1];
+    for (int32_t q = q0; q < q1; q++) {
+        const uint32_t v = pat.inc[q];
+        const int t = (int)(v >> 8);
+        int s = (int)((v >> 4) & 15u), s2 = (int)(v & 15u);
+        // The reference mirrors its edge-face blocks by transposition
+        // (fem/mth/tet.py:997-1004); with non-symmetric tensors the face-row /
+        // edge-column entries are therefore the edge-row / face-column formula.
+        const bool swp = TENSOR && (s >= 6) && (s2 < 6);
+        if (swp) { int tmp = s; s = s2; s2 = tmp; }
+        const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
+        V3 P[4];
+        const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) P[i] = {nodes[3 * (int64_t)vid[i]], nodes[3 * (int64_t)vid[i] + 1], nodes[3 * (int64_t)vid[i] + 2]};
+        const V3 e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]), e3 = sub(P[3], P[0]);
+        const V3 g1 = crs(e2, e3), g2 = crs(e3, e1), g3 = crs(e1, e2);
+        const V3 g0 = {-(g1.x + g2.x + g3.x), -(g1.y + g2.y + g3.y), -(g1.z + g2.z + g3.z)};
+        const double V6 = fabs(dot(e1, g1));
+#pragma unroll
+        for (int i = 0; i < 4; i++) st.put(i, P[i]);
+        st.put(4, g0);
+        st.put(5, g1);
+        st.put(6, g2);
+        st.put(7, g3);
+        int ia, ib, ic, ja, jb, jc;
+        slot_vertices(s, ia, ib, ic);
+        slot_vertices(s2, ja, jb, jc);
+        const bool fi = s >= 6, fj = s2 >= 6;
+        const V3 Gi[3] = {st.get(4 + ia), st.get(4 + ib), st.get(4 + ic)};
+        const V3 Gj[3] = {st.get(4 + ja), st.get(4 + jb), st.get(4 + jc)};
+        const V3 Pia = st.get(ia), Pib = st.get(ib), Pic = st.get(ic);
+        const V3 Pja = st.get(ja), Pjb = st.get(jb), Pjc = st.get(jc);
+        // Scales: edges L_ab (both modes); faces -L_ac (mode 1), +L_ab (mode 2).
+        const double Lia = nrm(sub(Pia, Pib)), Lja = nrm(sub(Pja, Pjb));
+        const double si[2] = {fi ? -nrm(sub(Pia, Pic)) : Lia, Lia};
+        const double sj[2] = {fj ? -nrm(sub(Pja, Pjc)) : Lja, Lja};
+        // Curl terms: K = (g_a x g_b, g_a x g_c, g_b x g_c), vertices paired
+        // with each K, and per-mode coefficients.
+        const V3 Ki[3] = {crs(Gi[0], Gi[1]), crs(Gi[0], Gi[2]), crs(Gi[1], Gi[2])};
+        const V3 Kj[3] = {crs(Gj[0], Gj[1]), crs(Gj[0], Gj[2]), crs(Gj[1], Gj[2])};
+        const int tvi0 = fi ? ic : ia, tvi2 = fi ? ia : ib;
+        const int tvj0 = fj ? jc : ja, tvj2 = fj ? ja : jb;
+        const int tvi[3] = {tvi0, ib, tvi2};
+        const int tvj[3] = {tvj0, jb, tvj2};
+        const double cfi[2][3] = {{fi ? -1.0 : -3.0, fi ? -2.0 : 0.0, fi ? -1.0 : 0.0},
+                                  {fi ? -2.0 : 0.0, fi ? -1.0 : -3.0, fi ? 1.0 : 0.0}};
+        const double cfj[2][3] = {{fj ? -1.0 : -3.0, fj ? -2.0 : 0.0, fj ? -1.0 : 0.0},
+                                  {fj ? -2.0 : 0.0, fj ? -1.0 : -3.0, fj ? 1.0 : 0.0}};
+        T H[3][3], Gb[3][3];
+        const cplx *Mm = TENSOR ? mat_m + 9 * (int64_t)t : nullptr;
+        const cplx *Ms = TENSOR ? mat_s + 9 * (int64_t)t : nullptr;
+        cplx mmt[9], mst[9];
+        if constexpr (TENSOR) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) { mmt[i] = Mm[i]; mst[i] = Ms[i]; }
+        }
+#pragma unroll
+        for (int x = 0; x < 3; x++)
+#pragma unroll
+            for (int y = 0; y < 3; y++) {
+                const double w = (tvi[x] == tvj[y]) ? 2.0 : 1.0;
+                if constexpr (TENSOR) {
+                    H[x][y] = qform(Ki[x], mst, Kj[y]) * w;
+                    Gb[x][y] = qform(Gi[x], mmt, Gj[y]);
+                } else {
+                    H[x][y] = dot(Ki[x], Kj[y]) * w;
+                    Gb[x][y] = dot(Gi[x], Gj[y]);
+                }
+            }
+        // Basis (p, q, r): r = a always; edge mode 1 (a, b, a), edge mode 2
+        // (b, b, a), face mode 1 (b, c, a), face mode 2 (c, b, a).  q sits at
+        // position c only for face mode 1.
+        const double dscale = 1.0 / (120.0 * V6 * V6 * V6);
+        const double fscale = 1.0 / (5040.0 * V6);
+        cplx cm = TENSOR ? cplx(1.0) : mat_m[t];
+        cplx cs = TENSOR ? cplx(1.0) : mat_s[t];
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const int p = (m == 0) ? (fi ? ib : ia) : (fi ? ic : ib);
+            const int qq = (m == 0 && fi) ? ic : ib;
+            const int r = ia;
+            const bool qc = (m == 0 && fi);
+#pragma unroll
+            for (int m2 = 0; m2 < 2; m2++) {
+                const int p2 = (m2 == 0) ? (fj ? jb : ja) : (fj ? jc : jb);
+                const int q2 = (m2 == 0 && fj) ? jc : jb;
+                const int r2 = ja;
+                const bool qc2 = (m2 == 0 && fj);
+                const T g_rq = qc2 ? Gb[0][2] : Gb[0][1];
+                const T g_qr = qc ? Gb[2][0] : Gb[1][0];
+                const T g_qq = qc ? (qc2 ? Gb[2][2] : Gb[2][1]) : (qc2 ? Gb[1][2] : Gb[1][1]);
+                // curl-curl
+                T d = T(0.0);
+#pragma unroll
+                for (int x = 0; x < 3; x++) {
+                    T row = T(0.0);
+#pragma unroll
+                    for (int y = 0; y < 3; y++) row += dmul(H[x][y], cfj[m2][y]);
+                    d += dmul(row, cfi[m][x]);
+                }
+                // mass (second coefficient: reference index choice unless exact)
+                double c2 = c4(p, qq, p2, r2);
+                if (!exact_face_mass && fi && fj && m2 == 0) c2 = c4(ib, r2, p2, q2);
+                T f = dmul(Gb[0][0], c4(p, qq, p2, q2));
+                f += dmul(g_rq, -c2);
+                f += dmul(g_qr, -c4(p, r, p2, q2));
+                f += dmul(g_qq, c4(p, r, p2, r2));
+                const double sij = si[m] * sj[m2];
+                cplx dv, fv;
+                if constexpr (TENSOR) {
+                    dv = d * (sij * dscale);
+                    fv = f * (sij * fscale);
+                } else {
+                    dv = cs * (d * sij * dscale);
+                    fv = cm * (f * sij * fscale);
+                }
+                const int a0 = swp ? m2 : m, a1 = swp ? m : m2;
+                accD[a0][a1] += dv;
+                accF[a0][a1] += fv;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int m2 = 0; m2 < 2; m2++) {
+            const int64_t idx = (int64_t)m * 2 * pat.nnz_ent + p0 + (int64_t)m2 * deg + k;
+            E[idx] = accD[m][m2];
+            B[idx] = accF[m][m2];
+        }
+}
+
+template <bool TENSOR>
+static int assemble(const emg_pattern *pat, const double *d_nodes, const int32_t *d_tets, const double *d_m,
+                    const double *d_s, int flags, double *d_E, double *d_B, void *stream) {
+    EMG_CHECK(pat && d_nodes && d_tets && d_m && d_s && d_E && d_B, "emg_assemble: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    unsigned grid = (unsigned)((pat->nnz_ent + ABLK - 1) / ABLK);
+    k_assemble<TENSOR><<<grid, ABLK, 0, st>>>(*pat, d_nodes, d_tets, reinterpret_cast<const cplx *>(d_m),
+                                              reinterpret_cast<const cplx *>(d_s), (flags & EMG_EXACT_FACE_MASS) ? 1 : 0,
+                                              reinterpret_cast<cplx *>(d_E), reinterpret_cast<cplx *>(d_B));
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, const int32_t *d_tets, const double *d_eps,
+                                const double *d_nu, int flags, double *d_E, double *d_B, void *stream) {
+    return assemble<false>(pat, d_nodes, d_tets, d_eps, d_nu, flags, d_E, d_B, stream);
+}
+
+extern "C" int emg_assemble_tensor(const emg_pattern *pat, const double *d_nodes, const int32_t *d_tets,
+                                   const double *d_mm, const double *d_ms, int flags, double *d_E, double *d_B,
+                                   void *stream) {
+    return assemble<true>(pat, d_nodes, d_tets, d_mm, d_ms, flags, d_E, d_B, stream);
+}

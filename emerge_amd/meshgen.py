@@ -1,0 +1,131 @@
+"""Structured tetrahedral meshes for the benchmark and test workloads.
+
+gmsh is not available, so every workload is meshed here: a box of nx*ny*nz
+hexahedra, each split into 6 tetrahedra around its main diagonal (Kuhn
+split, conforming across hexes), interior vertices jittered, all tets
+positively oriented (as gmsh's are).  Region tags come from tet centroids;
+the six box faces carry face tags 1..6 (xmin, xmax, ymin, ymax, zmin, zmax).
+"""
+from __future__ import annotations
+
+from itertools import permutations
+
+import numpy as np
+
+from .mesh import Mesh3D
+
+XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX = 1, 2, 3, 4, 5, 6
+
+
+def box_nodes(nx: int, ny: int, nz: int, size=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0)):
+    """(3, nv) grid vertices, x fastest; returns nodes and the (nx+1,ny+1,nz+1) id grid."""
+    xs = origin[0] + np.linspace(0, size[0], nx + 1)
+    ys = origin[1] + np.linspace(0, size[1], ny + 1)
+    zs = origin[2] + np.linspace(0, size[2], nz + 1)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), Z.ravel()])
+    ids = np.arange(nodes.shape[1]).reshape(nz + 1, ny + 1, nx + 1).transpose(2, 1, 0)
+    return nodes, ids
+
+
+def kuhn_tets(ids: np.ndarray) -> np.ndarray:
+    """(4, 6*nhex) tets of the Kuhn split of every hex of an id grid."""
+    nx, ny, nz = (s - 1 for s in ids.shape)
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = i.transpose(2, 1, 0).ravel(), j.transpose(2, 1, 0).ravel(), k.transpose(2, 1, 0).ravel()
+    out = []
+    unit = np.eye(3, dtype=np.int64)
+    for perm in permutations(range(3)):
+        p = np.zeros(3, dtype=np.int64)
+        verts = [ids[i, j, k]]
+        for ax in perm:
+            p = p + unit[ax]
+            verts.append(ids[i + p[0], j + p[1], k + p[2]])
+        out.append(np.stack(verts))
+    # interleave so the 6 tets of a hex are consecutive
+    return np.stack(out, axis=1).reshape(4, -1)
+
+
+def orient_positive(nodes: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    """Swap the last two vertices of negatively oriented tets."""
+    v = nodes[:, tets]
+    det = np.einsum("in,in->n", v[:, 1] - v[:, 0], np.cross((v[:, 2] - v[:, 0]).T, (v[:, 3] - v[:, 0]).T).T)
+    t = tets.copy()
+    neg = det < 0
+    t[2, neg], t[3, neg] = tets[3, neg], tets[2, neg]
+    return t
+
+
+def jitter_interior(nodes: np.ndarray, ids: np.ndarray, h: np.ndarray, frac: float, rng) -> np.ndarray:
+    """Move interior vertices by U(-frac, frac)*h per axis (boundary vertices stay)."""
+    nodes = nodes.copy()
+    inner = ids[1:-1, 1:-1, 1:-1].ravel()
+    d = rng.uniform(-frac, frac, size=(3, inner.shape[0])) * np.asarray(h)[:, None]
+    nodes[:, inner] += d
+    return nodes
+
+
+def box_face_tags(nodes: np.ndarray, tets: np.ndarray, size, origin=(0.0, 0.0, 0.0), tol=1e-9) -> dict[int, np.ndarray]:
+    """Boundary triangles of the box grouped by side (tags 1..6)."""
+    faces = np.concatenate([tets[[a, b, c]] for a, b, c in ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3))], axis=1)
+    s = np.sort(faces, axis=0)
+    key = np.ascontiguousarray(s.T).view([("", np.int64)] * 3).ravel()
+    _, idx, cnt = np.unique(key, return_index=True, return_counts=True)
+    bnd = s[:, idx[cnt == 1]]
+    out = {}
+    for tag, axis, val in ((XMIN, 0, origin[0]), (XMAX, 0, origin[0] + size[0]), (YMIN, 1, origin[1]),
+                           (YMAX, 1, origin[1] + size[1]), (ZMIN, 2, origin[2]), (ZMAX, 2, origin[2] + size[2])):
+        on = np.all(np.abs(nodes[axis, bnd] - val) < tol * max(size), axis=0)
+        out[tag] = bnd[:, on]
+    return out
+
+
+def structured_box(nx, ny, nz, size=(1.0, 1.0, 1.0), jitter=0.0, seed=0, origin=(0.0, 0.0, 0.0),
+                   keep=None, tag_fn=None) -> Mesh3D:
+    """Jittered Kuhn box mesh.
+
+    keep: optional predicate on tet centroids (3, nt) -> bool mask of tets to keep
+          (used to carve PEC holes).
+    tag_fn: optional centroid -> (nt,) int volume tags.
+    """
+    rng = np.random.default_rng(seed)
+    nodes, ids = box_nodes(nx, ny, nz, size, origin)
+    h = np.array([size[0] / nx, size[1] / ny, size[2] / nz])
+    if jitter > 0:
+        nodes = jitter_interior(nodes, ids, h, jitter, rng)
+    tets = orient_positive(nodes, kuhn_tets(ids))
+    cent = nodes[:, tets].mean(axis=1)
+    if keep is not None:
+        tets = tets[:, keep(cent)]
+        cent = nodes[:, tets].mean(axis=1)
+        used = np.unique(tets)
+        remap = -np.ones(nodes.shape[1], dtype=np.int64)
+        remap[used] = np.arange(used.shape[0])
+        nodes, tets = nodes[:, used], remap[tets]
+    tags = np.ones(tets.shape[1], dtype=np.int64) if tag_fn is None else tag_fn(cent)
+    faces = box_face_tags(nodes, tets, size, origin)
+    return Mesh3D(nodes, tets, tet_tags=tags, face_tags=faces)
+
+
+# ------------------------------------------------------------------ workloads
+def assembly_box(nx=50, ny=50, nz=67, seed=1):
+    """Assembly workload: unit box, nx*ny*nz hexes x 6 tets, interior vertices
+    jittered U(-0.1, 0.1)*h per axis; per tet eps_r' ~ U[1,10], tan(delta) ~
+    U[0, 0.02] (eps_r = eps_r'(1 - j tan(delta))), mu_r = 1.  Random numbers
+    are drawn in that order from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    nodes, ids = box_nodes(nx, ny, nz, (1.0, 1.0, 1.0))
+    h = np.array([1.0 / nx, 1.0 / ny, 1.0 / nz])
+    nodes = jitter_interior(nodes, ids, h, 0.1, rng)
+    tets = orient_positive(nodes, kuhn_tets(ids))
+    nt = tets.shape[1]
+    er = rng.uniform(1.0, 10.0, nt)
+    tand = rng.uniform(0.0, 0.02, nt)
+    eps = er * (1 - 1j * tand)
+    return Mesh3D(nodes, tets), eps
+
+
+def waveguide_wr90(nx=15, ny=7, nz=32, seed=0, length=50e-3):
+    """Air-filled WR-90 section 22.86 x 10.16 x `length` mm, ports on z faces."""
+    size = (22.86e-3, 10.16e-3, length)
+    return structured_box(nx, ny, nz, size, jitter=0.05, seed=seed)

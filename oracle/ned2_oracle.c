@@ -1,0 +1,320 @@
+/*
+ * CPU ORACLE -- test infrastructure only.
+ *
+ * Plain-C restatement of the reference's second-order Nedelec element
+ * kernels and the element loop that feeds the COO->CSR conversion.  It is
+ * the checker for the HIP kernels of emerge_amd and the host-core CPU
+ * baseline timed by bench.py.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product path never does.
+ *
+ * Each function cites the reference lines it restates.  The formulas are
+ * kept term for term (same index choices, same block mirroring, same matinv
+ * quirk) because parity is judged against exactly these expressions.
+ *
+ * Built by oracle/Makefile into oracle/liboracle.so (gcc -O2 -fopenmp).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+/* fem/mth/optimized.py:249-260  volume_coeff(a,b,c,d): slot 0 means "absent";
+ * result = prod(k_i!) / (sum k_i + 3)!  over slots 1..6.                    */
+static const double FACT[10] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880};
+static double VCOEFF[5][5][5][5];
+static double ACOEFF[5][5][5][5];
+static int coeff_ready = 0;
+
+static void init_coeffs(void) {
+    if (coeff_ready) return;
+    for (int a = 0; a < 5; a++)
+        for (int b = 0; b < 5; b++)
+            for (int c = 0; c < 5; c++)
+                for (int d = 0; d < 5; d++) {
+                    int k[7] = {0, 0, 0, 0, 0, 0, 0};
+                    k[a]++; k[b]++; k[c]++; k[d]++;
+                    int s = k[1] + k[2] + k[3] + k[4] + k[5] + k[6];
+                    double num = FACT[k[1]] * FACT[k[2]] * FACT[k[3]] * FACT[k[4]] * FACT[k[5]] * FACT[k[6]];
+                    VCOEFF[a][b][c][d] = num / FACT[s + 3];         /* optimized.py:258-259 */
+                    ACOEFF[a][b][c][d] = 2.0 * num / FACT[s + 2];   /* optimized.py:269-270 */
+                }
+    coeff_ready = 1;
+}
+
+/* fem/mth/optimized.py:343-369  tet_coefficients_bcd */
+static void tet_coefficients_bcd(const double *xs, const double *ys, const double *zs,
+                                 double *bb, double *cc, double *dd, double *V) {
+    double x1 = xs[0], x2 = xs[1], x3 = xs[2], x4 = xs[3];
+    double y1 = ys[0], y2 = ys[1], y3 = ys[2], y4 = ys[3];
+    double z1 = zs[0], z2 = zs[1], z3 = zs[2], z4 = zs[3];
+    *V = fabs(-x1*y2*z3/6 + x1*y2*z4/6 + x1*y3*z2/6 - x1*y3*z4/6 - x1*y4*z2/6 + x1*y4*z3/6
+              + x2*y1*z3/6 - x2*y1*z4/6 - x2*y3*z1/6 + x2*y3*z4/6 + x2*y4*z1/6 - x2*y4*z3/6
+              - x3*y1*z2/6 + x3*y1*z4/6 + x3*y2*z1/6 - x3*y2*z4/6 - x3*y4*z1/6 + x3*y4*z2/6
+              + x4*y1*z2/6 - x4*y1*z3/6 - x4*y2*z1/6 + x4*y2*z3/6 + x4*y3*z1/6 - x4*y3*z2/6);
+    bb[0] = -y2*z3 + y2*z4 + y3*z2 - y3*z4 - y4*z2 + y4*z3;
+    bb[1] = y1*z3 - y1*z4 - y3*z1 + y3*z4 + y4*z1 - y4*z3;
+    bb[2] = -y1*z2 + y1*z4 + y2*z1 - y2*z4 - y4*z1 + y4*z2;
+    bb[3] = y1*z2 - y1*z3 - y2*z1 + y2*z3 + y3*z1 - y3*z2;
+    cc[0] = x2*z3 - x2*z4 - x3*z2 + x3*z4 + x4*z2 - x4*z3;
+    cc[1] = -x1*z3 + x1*z4 + x3*z1 - x3*z4 - x4*z1 + x4*z3;
+    cc[2] = x1*z2 - x1*z4 - x2*z1 + x2*z4 + x4*z1 - x4*z2;
+    cc[3] = -x1*z2 + x1*z3 + x2*z1 - x2*z3 - x3*z1 + x3*z2;
+    dd[0] = -x2*y3 + x2*y4 + x3*y2 - x3*y4 - x4*y2 + x4*y3;
+    dd[1] = x1*y3 - x1*y4 - x3*y1 + x3*y4 + x4*y1 - x4*y3;
+    dd[2] = -x1*y2 + x1*y4 + x2*y1 - x2*y4 - x4*y1 + x4*y2;
+    dd[3] = x1*y2 - x1*y3 - x2*y1 + x2*y3 + x3*y1 - x3*y2;
+}
+
+/* fem/mth/optimized.py:428-436  compute_distances */
+static void compute_distances(const double *xs, const double *ys, const double *zs, int n, double *Ds) {
+    for (int i = 0; i < n; i++)
+        for (int j = i; j < n; j++) {
+            double v = sqrt((xs[i]-xs[j])*(xs[i]-xs[j]) + (ys[i]-ys[j])*(ys[i]-ys[j]) + (zs[i]-zs[j])*(zs[i]-zs[j]));
+            Ds[i*n + j] = v;
+            Ds[j*n + i] = v;
+        }
+}
+
+/* fem/mth/optimized.py:440-461  matinv.  The non-diagonal branch multiplies the
+ * adjugate by det instead of dividing (kept: the reference does this).       */
+void oracle_matinv(const cplx *s, cplx *out) {
+    for (int i = 0; i < 9; i++) out[i] = 0;
+    if (s[1] == 0 && s[2] == 0 && s[3] == 0 && s[5] == 0 && s[6] == 0 && s[7] == 0) {
+        out[0] = 1.0 / s[0];
+        out[4] = 1.0 / s[4];
+        out[8] = 1.0 / s[8];
+    } else {
+        cplx det = s[0]*s[4]*s[8] - s[0]*s[5]*s[7] - s[1]*s[3]*s[8] + s[1]*s[5]*s[6] + s[2]*s[3]*s[7] - s[2]*s[4]*s[6];
+        out[0] = s[4]*s[8] - s[5]*s[7];
+        out[1] = -s[1]*s[8] + s[2]*s[7];
+        out[2] = s[1]*s[5] - s[2]*s[4];
+        out[3] = -s[3]*s[8] + s[5]*s[6];
+        out[4] = s[0]*s[8] - s[2]*s[6];
+        out[5] = -s[0]*s[5] + s[2]*s[3];
+        out[6] = s[3]*s[7] - s[4]*s[6];
+        out[7] = -s[0]*s[7] + s[1]*s[6];
+        out[8] = s[0]*s[4] - s[1]*s[3];
+        for (int i = 0; i < 9; i++) out[i] *= det;
+    }
+}
+
+/* fem/mth/tet.py:34-41 matmul; optimized.py:219-225 cross_c; :245-247 dot_c */
+static void mv(const cplx *M, const cplx *v, cplx *o) {
+    o[0] = M[0]*v[0] + M[1]*v[1] + M[2]*v[2];
+    o[1] = M[3]*v[0] + M[4]*v[1] + M[5]*v[2];
+    o[2] = M[6]*v[0] + M[7]*v[1] + M[8]*v[2];
+}
+static void crs(const cplx *a, const cplx *b, cplx *o) {
+    o[0] = a[1]*b[2] - a[2]*b[1];
+    o[1] = a[2]*b[0] - a[0]*b[2];
+    o[2] = a[0]*b[1] - a[1]*b[0];
+}
+static cplx dt(const cplx *a, const cplx *b) { return a[0]*b[0] + a[1]*b[1] + a[2]*b[2]; }
+
+/* fem/mth/tet.py:854-1088  ned2_tet_stiff_mass.
+ * verts: (3,4) row-major [x0..x3, y0..y3, z0..z3]; edge_lengths (6);
+ * lem: local_edge_map (2,6) row-major; ltm: local_tri_map (3,4) row-major;
+ * Ms, Mm: (3,3) row-major.  D, F: (20,20) row-major outputs.               */
+void oracle_ned2_tet_stiff_mass(const double *verts, const double *edge_lengths,
+                                const int64_t *lem, const int64_t *ltm,
+                                const cplx *Ms, const cplx *Mm, cplx *D, cplx *F) {
+    init_coeffs();
+    const double *xs = verts, *ys = verts + 4, *zs = verts + 8;
+    double bb[4], cc[4], dd[4], V, Ds[16];
+    tet_coefficients_bcd(xs, ys, zs, bb, cc, dd, &V);                 /* :865 */
+    compute_distances(xs, ys, zs, 4, Ds);                             /* :870 */
+    cplx GL[4][3];
+    for (int k = 0; k < 4; k++) { GL[k][0] = bb[k]; GL[k][1] = cc[k]; GL[k][2] = dd[k]; }  /* :872-877 */
+    const int letters[6] = {1, 2, 3, 4, 5, 6};
+    const double KA = 1.0 / pow(6 * V, 4), KB = 1.0 / pow(6 * V, 2);    /* :881-882 */
+    const double V6 = 6 * V;
+#define VC(a, b, c, d) (VCOEFF[a][b][c][d] * V6)                       /* :886 */
+#define LEM(r, c) lem[(r) * 6 + (c)]
+#define LTM(r, c) ltm[(r) * 4 + (c)]
+    for (int i = 0; i < 400; i++) { D[i] = 0; F[i] = 0; }
+    for (int ei = 0; ei < 6; ei++) {                                  /* :887 */
+        int ei1 = (int)LEM(0, ei), ei2 = (int)LEM(1, ei);
+        const cplx *GA = GL[ei1], *GB = GL[ei2];
+        int A = letters[ei1], B = letters[ei2];
+        double L1 = edge_lengths[ei];
+        for (int ej = 0; ej < 6; ej++) {                              /* :895 */
+            int ej1 = (int)LEM(0, ej), ej2 = (int)LEM(1, ej);
+            int C = letters[ej1], Dl = letters[ej2];
+            const cplx *GC = GL[ej1], *GD = GL[ej2];
+            double VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
+            double VABCD = VC(A, B, C, Dl), VABCC = VC(A, B, C, C), VABDD = VC(A, B, Dl, Dl);
+            double VBBCD = VC(B, B, C, Dl), VAACD = VC(A, A, C, Dl), VAADD = VC(A, A, Dl, Dl);
+            double VBBCC = VC(B, B, C, C), VBBDD = VC(B, B, Dl, Dl), VAACC = VC(A, A, C, C);
+            double L2 = edge_lengths[ej];
+            cplx BB1[3], BC1[3];
+            mv(Mm, GC, BB1); mv(Mm, GD, BC1);                        /* :921-922 */
+            cplx BD1 = dt(GA, BB1), BE1 = dt(GA, BC1), BF1 = dt(GB, BB1), BG1 = dt(GB, BC1);
+            double Q2 = L1 * L2;
+            cplx x1[3], x2[3], mx2[3];
+            crs(GA, GB, x1); crs(GC, GD, x2); mv(Ms, x2, mx2);
+            cplx Q = Q2 * 9 * dt(x1, mx2);                            /* :929 */
+            D[(ei + 0) * 20 + ej + 0] = Q * VAC;
+            D[(ei + 0) * 20 + ej + 10] = Q * VAD;
+            D[(ei + 10) * 20 + ej + 0] = Q * VBC;
+            D[(ei + 10) * 20 + ej + 10] = Q * VBD;
+            F[(ei + 0) * 20 + ej + 0] = Q2 * (VABCD * BD1 - VABCC * BE1 - VAACD * BF1 + VAACC * BG1);
+            F[(ei + 0) * 20 + ej + 10] = Q2 * (VABDD * BD1 - VABCD * BE1 - VAADD * BF1 + VAACD * BG1);
+            F[(ei + 10) * 20 + ej + 0] = Q2 * (VBBCD * BD1 - VBBCC * BE1 - VABCD * BF1 + VABCC * BG1);
+            F[(ei + 10) * 20 + ej + 10] = Q2 * (VBBDD * BD1 - VBBCD * BE1 - VABDD * BF1 + VABCD * BG1);
+        }
+        for (int ej = 0; ej < 4; ej++) {                              /* :941 */
+            int ej1 = (int)LTM(0, ej), ej2 = (int)LTM(1, ej), fj = (int)LTM(2, ej);
+            int C = letters[ej1], Dl = letters[ej2], Fl = letters[fj];
+            const cplx *GC = GL[ej1], *GD = GL[ej2], *GF = GL[fj];
+            double VABCD = VC(A, B, C, Dl), VBBCD = VC(B, B, C, Dl);
+            double VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VAF = VC(A, Fl, 0, 0);
+            double VBF = VC(B, Fl, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
+            double VABDF = VC(A, B, Dl, Fl), VABCF = VC(A, B, Fl, C), VAADF = VC(A, A, Dl, Fl);
+            double VAACD = VC(A, A, C, Dl), VBBDF = VC(B, B, Dl, Fl);
+            double VBBCF = VC(B, B, Fl, C), VAACF = VC(A, A, C, Fl);
+            double Lab2 = Ds[ej1 * 4 + ej2], Lac2 = Ds[ej1 * 4 + fj];  /* :967-968 */
+            cplx AB1[3], t[3], mt[3];
+            crs(GA, GB, AB1);
+            crs(GC, GF, t); mv(Ms, t, mt); cplx AI1 = dt(AB1, mt);
+            crs(GD, GF, t); mv(Ms, t, mt); cplx AJ1 = dt(AB1, mt);
+            crs(GC, GD, t); mv(Ms, t, mt); cplx AK1 = dt(AB1, mt);
+            cplx BB1[3], BC1[3], BD1[3];
+            mv(Mm, GF, BB1); mv(Mm, GC, BC1); mv(Mm, GD, BD1);        /* :974-976 */
+            cplx BE1 = dt(GA, BB1), BF1 = dt(GA, BC1), BG1 = dt(GB, BB1), BH1 = dt(GB, BC1);
+            cplx BI1 = dt(GA, BD1), BJ1 = dt(GB, BD1);
+            D[(ei + 0) * 20 + ej + 6] = L1 * Lac2 * (-6 * VAD * AI1 - 3 * VAC * AJ1 - 3 * VAF * AK1);
+            D[(ei + 0) * 20 + ej + 16] = L1 * Lab2 * (6 * VAF * AK1 + 3 * VAD * AI1 - 3 * VAC * AJ1);
+            D[(ei + 10) * 20 + ej + 6] = L1 * Lac2 * (-6 * VBD * AI1 - 3 * VBC * AJ1 - 3 * VBF * AK1);
+            D[(ei + 10) * 20 + ej + 16] = L1 * Lab2 * (6 * VBF * AK1 + 3 * VBD * AI1 - 3 * VBC * AJ1);
+            F[(ei + 0) * 20 + ej + 6] = L1 * Lac2 * (VABCD * BE1 - VABDF * BF1 - VAACD * BG1 + VAADF * BH1);
+            F[(ei + 0) * 20 + ej + 16] = L1 * Lab2 * (VABDF * BF1 - VABCF * BI1 - VAADF * BH1 + VAACF * BJ1);
+            F[(ei + 10) * 20 + ej + 6] = L1 * Lac2 * (VBBCD * BE1 - VBBDF * BF1 - VABCD * BG1 + VABDF * BH1);
+            F[(ei + 10) * 20 + ej + 16] = L1 * Lab2 * (VBBDF * BF1 - VBBCF * BI1 - VABDF * BH1 + VABCF * BJ1);
+        }
+    }
+    /* :997-1004  mirror the edge-face blocks by transposition */
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 6; b++) {
+            D[(6 + a) * 20 + b] = D[b * 20 + 6 + a];
+            F[(6 + a) * 20 + b] = F[b * 20 + 6 + a];
+            D[(16 + a) * 20 + b] = D[b * 20 + 16 + a];
+            F[(16 + a) * 20 + b] = F[b * 20 + 16 + a];
+            D[(6 + a) * 20 + 10 + b] = D[(10 + b) * 20 + 6 + a];
+            F[(6 + a) * 20 + 10 + b] = F[(10 + b) * 20 + 6 + a];
+            D[(16 + a) * 20 + 10 + b] = D[(10 + b) * 20 + 16 + a];
+            F[(16 + a) * 20 + 10 + b] = F[(10 + b) * 20 + 16 + a];
+        }
+    for (int ei = 0; ei < 4; ei++) {                                  /* :1006 */
+        int ei1 = (int)LTM(0, ei), ei2 = (int)LTM(1, ei), fi = (int)LTM(2, ei);
+        int A = letters[ei1], B = letters[ei2], E = letters[fi];
+        const cplx *GA = GL[ei1], *GB = GL[ei2], *GE = GL[fi];
+        double Lac1 = Ds[ei1 * 4 + fi], Lab1 = Ds[ei1 * 4 + ei2];
+        for (int ej = 0; ej < 4; ej++) {
+            int ej1 = (int)LTM(0, ej), ej2 = (int)LTM(1, ej), fj = (int)LTM(2, ej);
+            int C = letters[ej1], Dl = letters[ej2], Fl = letters[fj];
+            const cplx *GC = GL[ej1], *GD = GL[ej2], *GF = GL[fj];
+            double VABCD = VC(A, B, C, Dl);
+            double VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VAF = VC(A, Fl, 0, 0);
+            double VBF = VC(B, Fl, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
+            double VDE = VC(E, Dl, 0, 0), VEF = VC(E, Fl, 0, 0), VCE = VC(E, C, 0, 0);
+            double VABDF = VC(A, B, Dl, Fl), VACEF = VC(A, C, E, Fl), VABCF = VC(A, B, Fl, C);
+            double VBCDE = VC(B, C, Dl, Fl), VBDEF = VC(B, E, Dl, Fl), VACDE = VC(E, A, C, Dl);
+            double VBCEF = VC(B, E, Fl, C), VADEF = VC(E, A, Dl, Fl);  /* :1023-1040 (indices as written) */
+            double Lac2 = Ds[ej1 * 4 + fj], Lab2 = Ds[ej1 * 4 + ej2];
+            cplx AB1[3], AF1[3], AG1[3], AH1[3], AI1[3], AJ1[3], t[3];
+            crs(GA, GE, AB1); crs(GB, GE, AF1); crs(GA, GB, AG1);
+            crs(GC, GF, t); mv(Ms, t, AH1);
+            crs(GD, GF, t); mv(Ms, t, AI1);
+            crs(GC, GD, t); mv(Ms, t, AJ1);
+            cplx AK1 = dt(AB1, AH1), AL1 = dt(AB1, AI1), AM1 = dt(AB1, AJ1);
+            cplx AN1 = dt(AF1, AH1), AO1 = dt(AF1, AI1), AP1 = dt(AF1, AJ1);
+            cplx AQ1 = dt(AG1, AH1), AR1 = dt(AG1, AI1), AS1 = dt(AG1, AJ1);
+            cplx BB1[3], BC1[3], BD1[3];
+            mv(Mm, GF, BB1); mv(Mm, GC, BC1); mv(Mm, GD, BD1);
+            cplx BE1 = dt(GE, BB1), BF1 = dt(GE, BC1), BG1 = dt(GA, BB1), BH1 = dt(GA, BC1);
+            cplx BI1 = dt(GE, BD1), BJ1 = dt(GA, BD1), BK1 = dt(GB, BB1), BL1 = dt(GB, BC1), BM1 = dt(GB, BD1);
+            cplx Q1 = 2 * VAD * AN1 + VAC * AO1 + VAF * AP1;
+            cplx Q2 = -2 * VAF * AP1 - VAD * AN1 + VAC * AO1;
+            D[(ei + 6) * 20 + ej + 6] = Lac1 * Lac2 * (4 * VBD * AK1 + 2 * VBC * AL1 + 2 * VBF * AM1 + Q1 + 2 * VDE * AQ1 + VCE * AR1 + VEF * AS1);
+            D[(ei + 6) * 20 + ej + 16] = Lac1 * Lab2 * (-4 * VBF * AM1 - 2 * VBD * AK1 + 2 * VBC * AL1 + Q2 - 2 * VEF * AS1 - VDE * AQ1 + VCE * AR1);
+            D[(ei + 16) * 20 + ej + 6] = Lab1 * Lac2 * (-4 * VDE * AQ1 - 2 * VCE * AR1 - 2 * VEF * AS1 - 2 * VBD * AK1 - VBC * AL1 - VBF * AM1 + Q1);
+            D[(ei + 16) * 20 + ej + 16] = Lab1 * Lab2 * (4 * VEF * AS1 + 2 * VDE * AQ1 - 2 * VCE * AR1 + 2 * VBF * AM1 + VBD * AK1 - VBC * AL1 + Q2);
+            F[(ei + 6) * 20 + ej + 6] = Lac1 * Lac2 * (VABCD * BE1 - VABDF * BF1 - VBCDE * BG1 + VBDEF * BH1);
+            F[(ei + 6) * 20 + ej + 16] = Lac1 * Lab2 * (VABDF * BF1 - VABCF * BI1 - VBDEF * BH1 + VBCEF * BJ1);
+            F[(ei + 16) * 20 + ej + 6] = Lab1 * Lac2 * (VBCDE * BG1 - VBDEF * BH1 - VACDE * BK1 + VADEF * BL1);
+            F[(ei + 16) * 20 + ej + 16] = Lab1 * Lab2 * (VBDEF * BH1 - VBCEF * BJ1 - VADEF * BL1 + VACEF * BM1);
+        }
+    }
+    for (int i = 0; i < 400; i++) { D[i] *= KA; F[i] *= KB; }       /* :1085-1086 */
+#undef VC
+#undef LEM
+#undef LTM
+}
+
+/* fem/mth/optimized.py:273-307  local_mapping (ndim rows, ncol columns) */
+static void local_mapping(const int64_t *vid, const int64_t *ids, int ndim, int ncol, int64_t *out) {
+    for (int t = 0; t < ncol; t++)
+        for (int j = 0; j < ndim; j++) {
+            out[j * ncol + t] = 0;
+            for (int k = 0; k < 4; k++)
+                if (vid[k] == ids[j * ncol + t]) { out[j * ncol + t] = k; break; }
+        }
+}
+
+/* fem/physics/edm/optimized_assembly.py:66-116  _matrix_builder.
+ * Array layouts follow the reference's (rows-first) numpy arrays:
+ * nodes (3,nv), tets (4,nt), tris (3,nf), edges (2,ne), tet_to_field (20,nt),
+ * tet_to_edge (6,nt); ur, er (3,3,nt) complex.  Outputs: 400*nt entries of
+ * dataE, dataB, rows, cols.  Runs over tets [t0, t1) only, so a bounded
+ * sample of a mesh can be timed.                                            */
+void oracle_matrix_builder(const double *nodes, int64_t nv, const int64_t *tets, int64_t nt,
+                           const int64_t *tris, int64_t nf, const int64_t *edges, int64_t ne,
+                           const double *all_edge_lengths, const int64_t *tet_to_field,
+                           const int64_t *tet_to_edge, const cplx *ur, const cplx *er,
+                           int64_t t0, int64_t t1,
+                           cplx *dataE, cplx *dataB, int64_t *rows, int64_t *cols) {
+    init_coeffs();
+    (void)nf;
+#pragma omp parallel for schedule(static)
+    for (int64_t itet = t0; itet < t1; itet++) {                      /* :88 prange */
+        int64_t p = (itet - t0) * 400;
+        cplx urt[9], ert[9], Ms[9];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                urt[a * 3 + b] = ur[(a * 3 + b) * nt + itet];
+                ert[a * 3 + b] = er[(a * 3 + b) * nt + itet];
+            }
+        int64_t vid[4];
+        for (int k = 0; k < 4; k++) vid[k] = tets[k * nt + itet];
+        /* :27-31 local_tet_to_triid and :33-36 local_tet_to_edgeid */
+        int64_t gtri[12], gedge[12], ltm[12], lem[12];
+        for (int j = 0; j < 4; j++) {
+            int64_t tri = tet_to_field[(6 + j) * nt + itet] - ne;
+            for (int r = 0; r < 3; r++) gtri[r * 4 + j] = tris[r * nf + tri];
+        }
+        for (int j = 0; j < 6; j++) {
+            int64_t e = tet_to_field[j * nt + itet];
+            for (int r = 0; r < 2; r++) gedge[r * 6 + j] = edges[r * ne + e];
+        }
+        local_mapping(vid, gtri, 3, 4, ltm);
+        local_mapping(vid, gedge, 2, 6, lem);
+        double el[6], verts[12];
+        for (int j = 0; j < 6; j++) el[j] = all_edge_lengths[tet_to_edge[j * nt + itet]];   /* :99 */
+        for (int r = 0; r < 3; r++)
+            for (int k = 0; k < 4; k++) verts[r * 4 + k] = nodes[r * nv + vid[k]];
+        oracle_matinv(urt, Ms);
+        cplx Esub[400], Bsub[400];
+        oracle_ned2_tet_stiff_mass(verts, el, lem, ltm, Ms, ert, Esub, Bsub);   /* :103-107 */
+        for (int ii = 0; ii < 20; ii++) {                              /* :109-112 */
+            int64_t idx = tet_to_field[ii * nt + itet];
+            for (int jj = 0; jj < 20; jj++) {
+                rows[p + 20 * ii + jj] = idx;
+                cols[p + ii + 20 * jj] = idx;
+            }
+        }
+        memcpy(dataE + p, Esub, sizeof(Esub));                         /* :114-115 */
+        memcpy(dataB + p, Bsub, sizeof(Bsub));
+    }
+}

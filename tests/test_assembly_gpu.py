@@ -1,0 +1,128 @@
+"""GPU assembly (HIP, through the C ABI) against the CPU oracle."""
+import numpy as np
+import pytest
+
+from emerge_amd import meshgen
+from emerge_amd.assembly import GPUPattern
+from oracle import oracle as O
+from tests.helpers import oracle_in_mesh_order, rel_fro, rel_max
+
+pytestmark = pytest.mark.gpu
+
+
+def _materials(nt, rng, tensor=False):
+    eps = rng.uniform(1, 10, nt) * (1 - 1j * rng.uniform(0, 0.02, nt))
+    er, ur = O.material_tensors(eps, 1.0 + 0.3 * rng.uniform(size=nt))
+    if tensor:
+        er = er + 0.4 * (rng.normal(size=er.shape) + 1j * rng.normal(size=er.shape))
+        ur = ur + 0.2 * rng.normal(size=ur.shape)
+    return er, ur
+
+
+def _check(mesh, er, ur, exact=False, order_seed=11):
+    pat = GPUPattern(mesh)
+    E, B = pat.assemble(er, ur, exact_face_mass=exact)
+    Eg, Bg = pat.to_scipy(E), pat.to_scipy(B)
+    om = O.OracleMesh(mesh.nodes, mesh.tets, order_seed=order_seed)
+    of = O.OracleNedelec2(om)
+    Eo, Bo = O.tet_mass_stiffness_matrices(of, er, ur)
+    Eo, Bo = oracle_in_mesh_order(mesh, Eo, Bo, of)
+    return Eg, Bg, Eo, Bo
+
+
+@pytest.mark.parametrize("shape,seed", [((1, 1, 1), 0), ((3, 2, 2), 1), ((4, 3, 5), 2)])
+def test_iso_matches_oracle(cuda, shape, seed):
+    print(f"iso shape={shape} seed={seed}", flush=True)
+    mesh = meshgen.structured_box(*shape, size=(1.0, 0.7, 1.3), jitter=0.1, seed=seed)
+    er, ur = _materials(mesh.n_tets, np.random.default_rng(seed))
+    Eg, Bg, Eo, Bo = _check(mesh, er, ur)
+    assert rel_fro(Eg, Eo) < 1e-12 and rel_max(Eg, Eo) < 1e-12
+    assert rel_fro(Bg, Bo) < 1e-12 and rel_max(Bg, Bo) < 1e-12
+    # CSR structure: sorted unique columns, pattern contains the oracle's
+    assert np.all(np.diff(Eg.indices[Eg.indptr[0]:Eg.indptr[1]]) > 0)
+    assert Eg.nnz == Eo.nnz or (abs(Eo) > 0).sum() <= Eg.nnz
+
+
+def test_tensor_matches_oracle(cuda):
+    print("tensor", flush=True)
+    mesh = meshgen.structured_box(2, 3, 2, size=(1.0, 1.0, 1.0), jitter=0.1, seed=5)
+    er, ur = _materials(mesh.n_tets, np.random.default_rng(5), tensor=True)
+    Eg, Bg, Eo, Bo = _check(mesh, er, ur)
+    assert rel_fro(Eg, Eo) < 1e-12 and rel_max(Eg, Eo) < 1e-12
+    assert rel_fro(Bg, Bo) < 1e-12 and rel_max(Bg, Bo) < 1e-12
+
+
+def test_exact_face_mass_is_symmetric(cuda):
+    print("exact face mass", flush=True)
+    mesh = meshgen.structured_box(2, 2, 2, jitter=0.1, seed=6)
+    er, ur = _materials(mesh.n_tets, np.random.default_rng(6))
+    pat = GPUPattern(mesh)
+    _, B0 = pat.assemble(er, ur)
+    _, B1 = pat.assemble(er, ur, exact_face_mass=True)
+    B0, B1 = pat.to_scipy(B0), pat.to_scipy(B1)
+    assert abs(B1 - B1.T).max() < 1e-14 * abs(B1).max()
+    assert abs(B0 - B0.T).max() > 1e-6 * abs(B0).max()     # the reference's is not
+    # they differ only in face-face columns of face mode 1
+    ne, nent = mesh.n_edges, mesh.n_entities
+    d = (B1 - B0).tocoo()
+    big = abs(d.data) > 1e-14 * abs(B1).max()
+    cols = d.col[big]
+    assert np.all((cols >= ne) & (cols < nent))
+
+
+def test_deterministic(cuda):
+    mesh = meshgen.structured_box(3, 3, 3, jitter=0.1, seed=8)
+    er, ur = _materials(mesh.n_tets, np.random.default_rng(8))
+    pat = GPUPattern(mesh)
+    E1, B1 = pat.assemble(er, ur)
+    E2, B2 = pat.assemble(er, ur)
+    assert bool((E1 == E2).all()) and bool((B1 == B2).all())
+
+
+def test_full_size_assembly_box(cuda):
+    """1,005,000-tet assembly workload: size-independent checks.
+
+    * rows of entities whose tets all lie inside a sub-box match the oracle
+      assembled on that sub-box alone (exact locality of assembly);
+    * the E block among those rows is complex-symmetric; no value is NaN.
+    """
+    import torch
+    print("full size 50x50x67", flush=True)
+    mesh, eps = meshgen.assembly_box()
+    er, ur = O.material_tensors(eps)
+    pat = GPUPattern(mesh)
+    assert pat.n == 6_457_334 and pat.nnz == 279_117_868
+    E, B = pat.assemble(er, ur)
+    torch.cuda.synchronize()
+    assert not bool(torch.isnan(E).any()) and not bool(torch.isnan(B).any())
+    Eh = pat.to_scipy(E)
+    Bh = pat.to_scipy(B)
+    del E, B
+    # sub-box: tets with centroid in [0.3,0.5]^2 x [0.3,0.45]
+    c = mesh.centers
+    sel = np.all((c >= np.array([[0.3], [0.3], [0.3]])) & (c <= np.array([[0.5], [0.5], [0.45]])), axis=0)
+    sub_tets = mesh.tets[:, sel]
+    om = O.OracleMesh(mesh.nodes, sub_tets)
+    of = O.OracleNedelec2(om)
+    Eo, Bo = O.tet_mass_stiffness_matrices(of, er[:, :, sel], ur[:, :, sel])
+    # entities all of whose tets are in the sub-box
+    inc_all = np.bincount(mesh.tet_ent.ravel(), minlength=mesh.n_entities)
+    inc_sub = np.bincount(mesh.tet_ent[sel].ravel(), minlength=mesh.n_entities)
+    inner = np.flatnonzero((inc_sub == inc_all) & (inc_all > 0))
+    assert inner.size > 100
+    nent = mesh.n_entities
+    rows_g = np.concatenate([inner, inner + nent])
+    from tests.helpers import mesh_dof_keys
+    o2m = O.permutation_to(of.dof_keys(), mesh_dof_keys(mesh))      # oracle DOF -> mesh DOF
+    m2o = -np.ones(pat.n, dtype=np.int64)
+    m2o[o2m] = np.arange(o2m.shape[0])
+    rows_o = m2o[rows_g]
+    assert np.all(rows_o >= 0)
+    from scipy.sparse import csr_matrix
+    for Mg, Mo in ((Eh, Eo), (Bh, Bo)):
+        g = Mg[rows_g]
+        o = Mo[rows_o].tocoo()
+        o2 = csr_matrix((o.data, (o.row, o2m[o.col])), shape=g.shape)
+        assert rel_fro(g, o2) < 1e-12
+    sub = Eh[rows_g][:, rows_g]
+    assert abs(sub - sub.T).max() < 1e-12 * abs(sub).max()

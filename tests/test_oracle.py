@@ -1,0 +1,125 @@
+"""Pins of the CPU oracle: closed-form element matrices vs brute-force
+quadrature of the reference's basis definitions, and helper closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import quadrature as Q
+
+EPAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (3, 1), (2, 3)]
+FTRIP = [(0, 1, 2), (0, 2, 3), (0, 3, 1), (1, 2, 3)]
+
+
+def _tet_case(rng):
+    verts = rng.normal(size=(3, 4))
+    gids = rng.permutation(60)[:4]
+    gedge = np.array([sorted([gids[i], gids[j]]) for i, j in EPAIRS]).T
+    gtri = np.array([sorted([gids[i], gids[j], gids[k]]) for i, j, k in FTRIP]).T
+    lem = O.local_mapping(gids, gedge)
+    ltm = O.local_mapping(gids, gtri)
+    el = np.array([np.linalg.norm(verts[:, i] - verts[:, j]) for i, j in EPAIRS])
+    return verts, gids, lem, ltm, el
+
+
+def _lam_int(verts, idx, n=6):
+    ref, w = Q.tet_rule(n)
+    T = np.stack([verts[:, 1] - verts[:, 0], verts[:, 2] - verts[:, 0], verts[:, 3] - verts[:, 0]], axis=1)
+    lam = Q.barycentric(verts, verts[:, :1] + T @ ref)
+    return np.sum(np.prod(lam[list(idx)], axis=0) * w) * abs(np.linalg.det(T))
+
+
+def _quirk_adjusted(verts, ltm, Mm, Fb):
+    """Exact mass matrix with the reference's coefficient of fem/mth/tet.py:1036
+    (VBCDE indexes the 4th vertex F instead of E) -- built from quadrature
+    moments, not from the factorial formula."""
+    G = Q.grad_barycentric(verts)
+    Fq = Fb.copy()
+    for ei in range(4):
+        a, b, e = ltm[:, ei]
+        Lac1 = np.linalg.norm(verts[:, a] - verts[:, e])
+        Lab1 = np.linalg.norm(verts[:, a] - verts[:, b])
+        for ej in range(4):
+            c, d, f = ltm[:, ej]
+            Lac2 = np.linalg.norm(verts[:, c] - verts[:, f])
+            delta = (_lam_int(verts, (b, c, d, f)) - _lam_int(verts, (b, c, d, e))) * (G[a] @ Mm @ G[f])
+            Fq[6 + ei, 6 + ej] += -Lac1 * Lac2 * delta
+            Fq[16 + ei, 6 + ej] += Lab1 * Lac2 * delta
+    return Fq
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_tet_element_vs_quadrature(case):
+    rng = np.random.default_rng(100 + case)
+    verts, gids, lem, ltm, el = _tet_case(rng)
+    if case < 2:
+        Mm = (2.5 - 0.3j) * np.eye(3)
+        Ms = np.eye(3) / (1.2 + 0.05j)
+    else:  # full (complex-symmetric) tensors
+        A = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        Mm = A + A.T + 6 * np.eye(3)
+        Bm = rng.normal(size=(3, 3))
+        Ms = Bm @ Bm.T + np.eye(3)
+    D, F = O.element_matrices(verts, el, lem, ltm, Ms, Mm)
+    Db, Fb = Q.brute_force_element(verts, gids, Ms, Mm)
+    assert np.abs(D - Db).max() / np.abs(Db).max() < 1e-12
+    Fq = _quirk_adjusted(verts, ltm, Mm, Fb)
+    assert np.abs(F - Fq).max() / np.abs(Fq).max() < 1e-12
+    # everything outside the two quirk blocks is the exact integral
+    mask = np.ones((20, 20), bool)
+    mask[6:10, 6:10] = False
+    mask[16:20, 6:10] = False
+    assert np.abs((F - Fb)[mask]).max() / np.abs(Fb).max() < 1e-12
+
+
+def test_reference_face_mass_differs_from_exact():
+    """The reference's face-face mass entries are not the exact integral; a
+    restatement that 'fixed' the index would fail here."""
+    rng = np.random.default_rng(7)
+    verts, gids, lem, ltm, el = _tet_case(rng)
+    D, F = O.element_matrices(verts, el, lem, ltm, np.eye(3), np.eye(3))
+    _, Fb = Q.brute_force_element(verts, gids, np.eye(3), np.eye(3))
+    assert np.abs(F[6:10, 6:10] - Fb[6:10, 6:10]).max() > 1e-3 * np.abs(Fb).max()
+
+
+def test_element_invariants():
+    rng = np.random.default_rng(3)
+    verts, gids, lem, ltm, el = _tet_case(rng)
+    D, F = O.element_matrices(verts, el, lem, ltm, np.eye(3), np.eye(3))
+    assert np.allclose(D, D.T, rtol=0, atol=1e-12 * abs(D).max())
+    # curl-curl null space is grad(P2) on one tet: 4 + 6 - 1 = 9, rank 11
+    s = np.linalg.svd(D.real, compute_uv=False)
+    assert np.sum(s > 1e-10 * s[0]) == 11
+    # constant field c: coefficients by least squares on sample points give x^T F x = |c|^2 V
+    pts = verts @ rng.dirichlet(np.ones(4), size=80).T
+    N = Q.ned2_tet_basis(verts, gids, pts)                 # (20,3,P)
+    c = np.array([0.3, -1.1, 0.7])
+    A = N.transpose(2, 1, 0).reshape(-1, 20)
+    x, *_ = np.linalg.lstsq(A, np.tile(c, pts.shape[1]), rcond=None)
+    T = np.stack([verts[:, 1] - verts[:, 0], verts[:, 2] - verts[:, 0], verts[:, 3] - verts[:, 0]], axis=1)
+    V = abs(np.linalg.det(T)) / 6
+    _, Fb = Q.brute_force_element(verts, gids, np.eye(3), np.eye(3))
+    assert abs(x @ Fb.real @ x - c @ c * V) < 1e-10 * V
+    assert abs(x @ D.real @ x) < 1e-9 * V                 # curl of a constant field is 0
+
+
+def test_volume_coeff_closed_form():
+    rng = np.random.default_rng(1)
+    verts = rng.normal(size=(3, 4))
+    T = np.stack([verts[:, 1] - verts[:, 0], verts[:, 2] - verts[:, 0], verts[:, 3] - verts[:, 0]], axis=1)
+    V = abs(np.linalg.det(T)) / 6
+    for idx in [(0, 1, 2, 3), (0, 0, 1, 2), (1, 1, 1, 1), (2, 3, 3, 0), (0, 0, 0, 3)]:
+        cnt = [idx.count(k) for k in range(4)]
+        closed = math.prod(math.factorial(c) for c in cnt) * 6 * V / math.factorial(7)
+        assert abs(_lam_int(verts, idx) - closed) < 1e-13 * V
+
+
+def test_matinv_reference_branches():
+    d = np.diag([2.0 + 1j, 3.0, 0.5]).astype(complex)
+    assert np.allclose(O.matinv(d), np.linalg.inv(d))
+    rng = np.random.default_rng(2)
+    s = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    # fem/mth/optimized.py:449-460 multiplies the adjugate by det: det^2 * inv
+    det = np.linalg.det(s)
+    assert np.allclose(O.matinv(s), np.linalg.inv(s) * det * det)
