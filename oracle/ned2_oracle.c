@@ -14,17 +14,29 @@
  * Built by oracle/Makefile into oracle/liboracle.so (gcc -O2 -fopenmp).
  */
 #include <complex.h>
-#include <math.h>
 #include <stdint.h>
 #include <string.h>
+#include <tgmath.h>
 
-typedef double complex cplx;
+/* Compiled twice: in double (the restatement proper) and, with ORACLE_LD, in
+ * x87 long double, which evaluates the same algorithm without the fp64
+ * cancellation of its absolute-coordinate cofactor/volume formulas.
+ * Entry points keep double inputs/outputs; the _ld build suffixes its names. */
+#ifdef ORACLE_LD
+#define REAL long double
+#define SFX(n) n##_ld
+#else
+#define REAL double
+#define SFX(n) n
+#endif
+typedef REAL real;
+typedef REAL complex cplx;
+typedef double complex cplx_io;
 
 /* fem/mth/optimized.py:249-260  volume_coeff(a,b,c,d): slot 0 means "absent";
  * result = prod(k_i!) / (sum k_i + 3)!  over slots 1..6.                    */
-static const double FACT[10] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880};
-static double VCOEFF[5][5][5][5];
-static double ACOEFF[5][5][5][5];
+static const real FACT[10] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880};
+static real VCOEFF[5][5][5][5];
 static int coeff_ready = 0;
 
 static void init_coeffs(void) {
@@ -36,19 +48,18 @@ static void init_coeffs(void) {
                     int k[7] = {0, 0, 0, 0, 0, 0, 0};
                     k[a]++; k[b]++; k[c]++; k[d]++;
                     int s = k[1] + k[2] + k[3] + k[4] + k[5] + k[6];
-                    double num = FACT[k[1]] * FACT[k[2]] * FACT[k[3]] * FACT[k[4]] * FACT[k[5]] * FACT[k[6]];
+                    real num = FACT[k[1]] * FACT[k[2]] * FACT[k[3]] * FACT[k[4]] * FACT[k[5]] * FACT[k[6]];
                     VCOEFF[a][b][c][d] = num / FACT[s + 3];         /* optimized.py:258-259 */
-                    ACOEFF[a][b][c][d] = 2.0 * num / FACT[s + 2];   /* optimized.py:269-270 */
                 }
     coeff_ready = 1;
 }
 
 /* fem/mth/optimized.py:343-369  tet_coefficients_bcd */
-static void tet_coefficients_bcd(const double *xs, const double *ys, const double *zs,
-                                 double *bb, double *cc, double *dd, double *V) {
-    double x1 = xs[0], x2 = xs[1], x3 = xs[2], x4 = xs[3];
-    double y1 = ys[0], y2 = ys[1], y3 = ys[2], y4 = ys[3];
-    double z1 = zs[0], z2 = zs[1], z3 = zs[2], z4 = zs[3];
+static void tet_coefficients_bcd(const real *xs, const real *ys, const real *zs,
+                                 real *bb, real *cc, real *dd, real *V) {
+    real x1 = xs[0], x2 = xs[1], x3 = xs[2], x4 = xs[3];
+    real y1 = ys[0], y2 = ys[1], y3 = ys[2], y4 = ys[3];
+    real z1 = zs[0], z2 = zs[1], z3 = zs[2], z4 = zs[3];
     *V = fabs(-x1*y2*z3/6 + x1*y2*z4/6 + x1*y3*z2/6 - x1*y3*z4/6 - x1*y4*z2/6 + x1*y4*z3/6
               + x2*y1*z3/6 - x2*y1*z4/6 - x2*y3*z1/6 + x2*y3*z4/6 + x2*y4*z1/6 - x2*y4*z3/6
               - x3*y1*z2/6 + x3*y1*z4/6 + x3*y2*z1/6 - x3*y2*z4/6 - x3*y4*z1/6 + x3*y4*z2/6
@@ -68,10 +79,10 @@ static void tet_coefficients_bcd(const double *xs, const double *ys, const doubl
 }
 
 /* fem/mth/optimized.py:428-436  compute_distances */
-static void compute_distances(const double *xs, const double *ys, const double *zs, int n, double *Ds) {
+static void compute_distances(const real *xs, const real *ys, const real *zs, int n, real *Ds) {
     for (int i = 0; i < n; i++)
         for (int j = i; j < n; j++) {
-            double v = sqrt((xs[i]-xs[j])*(xs[i]-xs[j]) + (ys[i]-ys[j])*(ys[i]-ys[j]) + (zs[i]-zs[j])*(zs[i]-zs[j]));
+            real v = sqrt((xs[i]-xs[j])*(xs[i]-xs[j]) + (ys[i]-ys[j])*(ys[i]-ys[j]) + (zs[i]-zs[j])*(zs[i]-zs[j]));
             Ds[i*n + j] = v;
             Ds[j*n + i] = v;
         }
@@ -79,7 +90,7 @@ static void compute_distances(const double *xs, const double *ys, const double *
 
 /* fem/mth/optimized.py:440-461  matinv.  The non-diagonal branch multiplies the
  * adjugate by det instead of dividing (kept: the reference does this).       */
-void oracle_matinv(const cplx *s, cplx *out) {
+static void matinv(const cplx *s, cplx *out) {
     for (int i = 0; i < 9; i++) out[i] = 0;
     if (s[1] == 0 && s[2] == 0 && s[3] == 0 && s[5] == 0 && s[6] == 0 && s[7] == 0) {
         out[0] = 1.0 / s[0];
@@ -117,19 +128,19 @@ static cplx dt(const cplx *a, const cplx *b) { return a[0]*b[0] + a[1]*b[1] + a[
  * verts: (3,4) row-major [x0..x3, y0..y3, z0..z3]; edge_lengths (6);
  * lem: local_edge_map (2,6) row-major; ltm: local_tri_map (3,4) row-major;
  * Ms, Mm: (3,3) row-major.  D, F: (20,20) row-major outputs.               */
-void oracle_ned2_tet_stiff_mass(const double *verts, const double *edge_lengths,
-                                const int64_t *lem, const int64_t *ltm,
-                                const cplx *Ms, const cplx *Mm, cplx *D, cplx *F) {
+static void stiff_mass(const real *verts, const real *edge_lengths,
+                       const int64_t *lem, const int64_t *ltm,
+                       const cplx *Ms, const cplx *Mm, cplx *D, cplx *F) {
     init_coeffs();
-    const double *xs = verts, *ys = verts + 4, *zs = verts + 8;
-    double bb[4], cc[4], dd[4], V, Ds[16];
+    const real *xs = verts, *ys = verts + 4, *zs = verts + 8;
+    real bb[4], cc[4], dd[4], V, Ds[16];
     tet_coefficients_bcd(xs, ys, zs, bb, cc, dd, &V);                 /* :865 */
     compute_distances(xs, ys, zs, 4, Ds);                             /* :870 */
     cplx GL[4][3];
     for (int k = 0; k < 4; k++) { GL[k][0] = bb[k]; GL[k][1] = cc[k]; GL[k][2] = dd[k]; }  /* :872-877 */
     const int letters[6] = {1, 2, 3, 4, 5, 6};
-    const double KA = 1.0 / pow(6 * V, 4), KB = 1.0 / pow(6 * V, 2);    /* :881-882 */
-    const double V6 = 6 * V;
+    const real KA = 1 / pow(6 * V, 4), KB = 1 / pow(6 * V, 2);    /* :881-882 */
+    const real V6 = 6 * V;
 #define VC(a, b, c, d) (VCOEFF[a][b][c][d] * V6)                       /* :886 */
 #define LEM(r, c) lem[(r) * 6 + (c)]
 #define LTM(r, c) ltm[(r) * 4 + (c)]
@@ -138,20 +149,20 @@ void oracle_ned2_tet_stiff_mass(const double *verts, const double *edge_lengths,
         int ei1 = (int)LEM(0, ei), ei2 = (int)LEM(1, ei);
         const cplx *GA = GL[ei1], *GB = GL[ei2];
         int A = letters[ei1], B = letters[ei2];
-        double L1 = edge_lengths[ei];
+        real L1 = edge_lengths[ei];
         for (int ej = 0; ej < 6; ej++) {                              /* :895 */
             int ej1 = (int)LEM(0, ej), ej2 = (int)LEM(1, ej);
             int C = letters[ej1], Dl = letters[ej2];
             const cplx *GC = GL[ej1], *GD = GL[ej2];
-            double VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
-            double VABCD = VC(A, B, C, Dl), VABCC = VC(A, B, C, C), VABDD = VC(A, B, Dl, Dl);
-            double VBBCD = VC(B, B, C, Dl), VAACD = VC(A, A, C, Dl), VAADD = VC(A, A, Dl, Dl);
-            double VBBCC = VC(B, B, C, C), VBBDD = VC(B, B, Dl, Dl), VAACC = VC(A, A, C, C);
-            double L2 = edge_lengths[ej];
+            real VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
+            real VABCD = VC(A, B, C, Dl), VABCC = VC(A, B, C, C), VABDD = VC(A, B, Dl, Dl);
+            real VBBCD = VC(B, B, C, Dl), VAACD = VC(A, A, C, Dl), VAADD = VC(A, A, Dl, Dl);
+            real VBBCC = VC(B, B, C, C), VBBDD = VC(B, B, Dl, Dl), VAACC = VC(A, A, C, C);
+            real L2 = edge_lengths[ej];
             cplx BB1[3], BC1[3];
             mv(Mm, GC, BB1); mv(Mm, GD, BC1);                        /* :921-922 */
             cplx BD1 = dt(GA, BB1), BE1 = dt(GA, BC1), BF1 = dt(GB, BB1), BG1 = dt(GB, BC1);
-            double Q2 = L1 * L2;
+            real Q2 = L1 * L2;
             cplx x1[3], x2[3], mx2[3];
             crs(GA, GB, x1); crs(GC, GD, x2); mv(Ms, x2, mx2);
             cplx Q = Q2 * 9 * dt(x1, mx2);                            /* :929 */
@@ -168,13 +179,13 @@ void oracle_ned2_tet_stiff_mass(const double *verts, const double *edge_lengths,
             int ej1 = (int)LTM(0, ej), ej2 = (int)LTM(1, ej), fj = (int)LTM(2, ej);
             int C = letters[ej1], Dl = letters[ej2], Fl = letters[fj];
             const cplx *GC = GL[ej1], *GD = GL[ej2], *GF = GL[fj];
-            double VABCD = VC(A, B, C, Dl), VBBCD = VC(B, B, C, Dl);
-            double VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VAF = VC(A, Fl, 0, 0);
-            double VBF = VC(B, Fl, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
-            double VABDF = VC(A, B, Dl, Fl), VABCF = VC(A, B, Fl, C), VAADF = VC(A, A, Dl, Fl);
-            double VAACD = VC(A, A, C, Dl), VBBDF = VC(B, B, Dl, Fl);
-            double VBBCF = VC(B, B, Fl, C), VAACF = VC(A, A, C, Fl);
-            double Lab2 = Ds[ej1 * 4 + ej2], Lac2 = Ds[ej1 * 4 + fj];  /* :967-968 */
+            real VABCD = VC(A, B, C, Dl), VBBCD = VC(B, B, C, Dl);
+            real VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VAF = VC(A, Fl, 0, 0);
+            real VBF = VC(B, Fl, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
+            real VABDF = VC(A, B, Dl, Fl), VABCF = VC(A, B, Fl, C), VAADF = VC(A, A, Dl, Fl);
+            real VAACD = VC(A, A, C, Dl), VBBDF = VC(B, B, Dl, Fl);
+            real VBBCF = VC(B, B, Fl, C), VAACF = VC(A, A, C, Fl);
+            real Lab2 = Ds[ej1 * 4 + ej2], Lac2 = Ds[ej1 * 4 + fj];  /* :967-968 */
             cplx AB1[3], t[3], mt[3];
             crs(GA, GB, AB1);
             crs(GC, GF, t); mv(Ms, t, mt); cplx AI1 = dt(AB1, mt);
@@ -210,19 +221,19 @@ void oracle_ned2_tet_stiff_mass(const double *verts, const double *edge_lengths,
         int ei1 = (int)LTM(0, ei), ei2 = (int)LTM(1, ei), fi = (int)LTM(2, ei);
         int A = letters[ei1], B = letters[ei2], E = letters[fi];
         const cplx *GA = GL[ei1], *GB = GL[ei2], *GE = GL[fi];
-        double Lac1 = Ds[ei1 * 4 + fi], Lab1 = Ds[ei1 * 4 + ei2];
+        real Lac1 = Ds[ei1 * 4 + fi], Lab1 = Ds[ei1 * 4 + ei2];
         for (int ej = 0; ej < 4; ej++) {
             int ej1 = (int)LTM(0, ej), ej2 = (int)LTM(1, ej), fj = (int)LTM(2, ej);
             int C = letters[ej1], Dl = letters[ej2], Fl = letters[fj];
             const cplx *GC = GL[ej1], *GD = GL[ej2], *GF = GL[fj];
-            double VABCD = VC(A, B, C, Dl);
-            double VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VAF = VC(A, Fl, 0, 0);
-            double VBF = VC(B, Fl, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
-            double VDE = VC(E, Dl, 0, 0), VEF = VC(E, Fl, 0, 0), VCE = VC(E, C, 0, 0);
-            double VABDF = VC(A, B, Dl, Fl), VACEF = VC(A, C, E, Fl), VABCF = VC(A, B, Fl, C);
-            double VBCDE = VC(B, C, Dl, Fl), VBDEF = VC(B, E, Dl, Fl), VACDE = VC(E, A, C, Dl);
-            double VBCEF = VC(B, E, Fl, C), VADEF = VC(E, A, Dl, Fl);  /* :1023-1040 (indices as written) */
-            double Lac2 = Ds[ej1 * 4 + fj], Lab2 = Ds[ej1 * 4 + ej2];
+            real VABCD = VC(A, B, C, Dl);
+            real VAD = VC(A, Dl, 0, 0), VAC = VC(A, C, 0, 0), VAF = VC(A, Fl, 0, 0);
+            real VBF = VC(B, Fl, 0, 0), VBC = VC(B, C, 0, 0), VBD = VC(B, Dl, 0, 0);
+            real VDE = VC(E, Dl, 0, 0), VEF = VC(E, Fl, 0, 0), VCE = VC(E, C, 0, 0);
+            real VABDF = VC(A, B, Dl, Fl), VACEF = VC(A, C, E, Fl), VABCF = VC(A, B, Fl, C);
+            real VBCDE = VC(B, C, Dl, Fl), VBDEF = VC(B, E, Dl, Fl), VACDE = VC(E, A, C, Dl);
+            real VBCEF = VC(B, E, Fl, C), VADEF = VC(E, A, Dl, Fl);  /* :1023-1040 (indices as written) */
+            real Lac2 = Ds[ej1 * 4 + fj], Lab2 = Ds[ej1 * 4 + ej2];
             cplx AB1[3], AF1[3], AG1[3], AH1[3], AI1[3], AJ1[3], t[3];
             crs(GA, GE, AB1); crs(GB, GE, AF1); crs(GA, GB, AG1);
             crs(GC, GF, t); mv(Ms, t, AH1);
@@ -253,6 +264,26 @@ void oracle_ned2_tet_stiff_mass(const double *verts, const double *edge_lengths,
 #undef LTM
 }
 
+/* Public single-element entry points (double in/out). */
+void SFX(oracle_matinv)(const cplx_io *s, cplx_io *out) {
+    cplx a[9], o[9];
+    for (int i = 0; i < 9; i++) a[i] = s[i];
+    matinv(a, o);
+    for (int i = 0; i < 9; i++) out[i] = (cplx_io)o[i];
+}
+
+void SFX(oracle_ned2_tet_stiff_mass)(const double *verts, const double *edge_lengths,
+                                     const int64_t *lem, const int64_t *ltm,
+                                     const cplx_io *Ms, const cplx_io *Mm, cplx_io *D, cplx_io *F) {
+    real v[12], el[6];
+    cplx ms[9], mm[9], d[400], f[400];
+    for (int i = 0; i < 12; i++) v[i] = verts[i];
+    for (int i = 0; i < 6; i++) el[i] = edge_lengths[i];
+    for (int i = 0; i < 9; i++) { ms[i] = Ms[i]; mm[i] = Mm[i]; }
+    stiff_mass(v, el, lem, ltm, ms, mm, d, f);
+    for (int i = 0; i < 400; i++) { D[i] = (cplx_io)d[i]; F[i] = (cplx_io)f[i]; }
+}
+
 /* fem/mth/optimized.py:273-307  local_mapping (ndim rows, ncol columns) */
 static void local_mapping(const int64_t *vid, const int64_t *ids, int ndim, int ncol, int64_t *out) {
     for (int t = 0; t < ncol; t++)
@@ -269,12 +300,12 @@ static void local_mapping(const int64_t *vid, const int64_t *ids, int ndim, int 
  * tet_to_edge (6,nt); ur, er (3,3,nt) complex.  Outputs: 400*nt entries of
  * dataE, dataB, rows, cols.  Runs over tets [t0, t1) only, so a bounded
  * sample of a mesh can be timed.                                            */
-void oracle_matrix_builder(const double *nodes, int64_t nv, const int64_t *tets, int64_t nt,
-                           const int64_t *tris, int64_t nf, const int64_t *edges, int64_t ne,
-                           const double *all_edge_lengths, const int64_t *tet_to_field,
-                           const int64_t *tet_to_edge, const cplx *ur, const cplx *er,
-                           int64_t t0, int64_t t1,
-                           cplx *dataE, cplx *dataB, int64_t *rows, int64_t *cols) {
+void SFX(oracle_matrix_builder)(const double *nodes, int64_t nv, const int64_t *tets, int64_t nt,
+                                const int64_t *tris, int64_t nf, const int64_t *edges, int64_t ne,
+                                const double *all_edge_lengths, const int64_t *tet_to_field,
+                                const int64_t *tet_to_edge, const cplx_io *ur, const cplx_io *er,
+                                int64_t t0, int64_t t1,
+                                cplx_io *dataE, cplx_io *dataB, int64_t *rows, int64_t *cols) {
     init_coeffs();
     (void)nf;
 #pragma omp parallel for schedule(static)
@@ -300,13 +331,13 @@ void oracle_matrix_builder(const double *nodes, int64_t nv, const int64_t *tets,
         }
         local_mapping(vid, gtri, 3, 4, ltm);
         local_mapping(vid, gedge, 2, 6, lem);
-        double el[6], verts[12];
+        real el[6], verts[12];
         for (int j = 0; j < 6; j++) el[j] = all_edge_lengths[tet_to_edge[j * nt + itet]];   /* :99 */
         for (int r = 0; r < 3; r++)
             for (int k = 0; k < 4; k++) verts[r * 4 + k] = nodes[r * nv + vid[k]];
-        oracle_matinv(urt, Ms);
+        matinv(urt, Ms);
         cplx Esub[400], Bsub[400];
-        oracle_ned2_tet_stiff_mass(verts, el, lem, ltm, Ms, ert, Esub, Bsub);   /* :103-107 */
+        stiff_mass(verts, el, lem, ltm, Ms, ert, Esub, Bsub);   /* :103-107 */
         for (int ii = 0; ii < 20; ii++) {                              /* :109-112 */
             int64_t idx = tet_to_field[ii * nt + itet];
             for (int jj = 0; jj < 20; jj++) {
@@ -314,7 +345,9 @@ void oracle_matrix_builder(const double *nodes, int64_t nv, const int64_t *tets,
                 cols[p + ii + 20 * jj] = idx;
             }
         }
-        memcpy(dataE + p, Esub, sizeof(Esub));                         /* :114-115 */
-        memcpy(dataB + p, Bsub, sizeof(Bsub));
+        for (int i = 0; i < 400; i++) {                                /* :114-115 */
+            dataE[p + i] = (cplx_io)Esub[i];
+            dataB[p + i] = (cplx_io)Bsub[i];
+        }
     }
 }

@@ -41,10 +41,13 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(path)
         P = ctypes.c_void_p
         I64 = ctypes.c_int64
-        L.oracle_matrix_builder.argtypes = [P, I64, P, I64, P, I64, P, I64, P, P, P, P, P, I64, I64, P, P, P, P]
-        L.oracle_matrix_builder.restype = None
-        L.oracle_ned2_tet_stiff_mass.argtypes = [P] * 8
-        L.oracle_ned2_tet_stiff_mass.restype = None
+        for sfx in ("", "_ld"):
+            fn = getattr(L, "oracle_matrix_builder" + sfx)
+            fn.argtypes = [P, I64, P, I64, P, I64, P, I64, P, P, P, P, P, I64, I64, P, P, P, P]
+            fn.restype = None
+            fn = getattr(L, "oracle_ned2_tet_stiff_mass" + sfx)
+            fn.argtypes = [P] * 8
+            fn.restype = None
         L.oracle_matinv.argtypes = [P, P]
         L.oracle_matinv.restype = None
         L.oracle_ned2_tri_stiff_force.argtypes = [P, ctypes.c_double * 2, P, P, ctypes.c_int, P, P]
@@ -194,8 +197,13 @@ def material_tensors(eps: np.ndarray, mu: np.ndarray | None = None) -> tuple[np.
     return er, ur
 
 
-def matrix_builder(field: OracleNedelec2, er: np.ndarray, ur: np.ndarray, t0: int = 0, t1: int | None = None):
-    """fem/physics/edm/optimized_assembly.py:66-116 over tets [t0, t1)."""
+def matrix_builder(field: OracleNedelec2, er: np.ndarray, ur: np.ndarray, t0: int = 0, t1: int | None = None,
+                   extended: bool = False):
+    """fem/physics/edm/optimized_assembly.py:66-116 over tets [t0, t1).
+
+    extended=True evaluates the same algorithm in long double (the reference's
+    absolute-coordinate volume/cofactor formulas lose digits to cancellation
+    in fp64 on fine meshes away from the origin)."""
     m = field.mesh
     nt = m.n_tets
     t1 = nt if t1 is None else t1
@@ -206,16 +214,17 @@ def matrix_builder(field: OracleNedelec2, er: np.ndarray, ur: np.ndarray, t0: in
     cols = np.empty(n, dtype=np.int64)
     er = np.ascontiguousarray(er, dtype=np.complex128)
     ur = np.ascontiguousarray(ur, dtype=np.complex128)
-    lib().oracle_matrix_builder(_p(m.nodes), m.nodes.shape[1], _p(m.tets), nt, _p(m.tris), m.n_tris,
+    fn = lib().oracle_matrix_builder_ld if extended else lib().oracle_matrix_builder
+    fn(_p(m.nodes), m.nodes.shape[1], _p(m.tets), nt, _p(m.tris), m.n_tris,
                                 _p(m.edges), m.n_edges, _p(m.edge_lengths), _p(field.tet_to_field),
                                 _p(np.ascontiguousarray(m.tet_to_edge)), _p(ur), _p(er), t0, t1,
                                 _p(dataE), _p(dataB), _p(rows), _p(cols))
     return dataE, dataB, rows, cols
 
 
-def tet_mass_stiffness_matrices(field: OracleNedelec2, er: np.ndarray, ur: np.ndarray):
+def tet_mass_stiffness_matrices(field: OracleNedelec2, er: np.ndarray, ur: np.ndarray, extended: bool = False):
     """fem/physics/edm/optimized_assembly.py:43-64: E (curl-curl) and B (mass) CSR."""
-    dataE, dataB, rows, cols = matrix_builder(field, er, ur)
+    dataE, dataB, rows, cols = matrix_builder(field, er, ur, extended=extended)
     N = field.n_field
     E = sp.coo_matrix((dataE, (rows, cols)), shape=(N, N)).tocsr()
     B = sp.coo_matrix((dataB, (rows, cols)), shape=(N, N)).tocsr()

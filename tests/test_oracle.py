@@ -123,3 +123,24 @@ def test_matinv_reference_branches():
     # fem/mth/optimized.py:449-460 multiplies the adjugate by det: det^2 * inv
     det = np.linalg.det(s)
     assert np.allclose(O.matinv(s), np.linalg.inv(s) * det * det)
+
+
+def test_extended_oracle_pins():
+    """The long-double build evaluates the same algorithm: equal to the fp64
+    build on a well-conditioned mesh, and translation invariant where the
+    fp64 build is not."""
+    from emerge_amd import meshgen
+    from scipy.sparse.linalg import norm
+    mesh = meshgen.structured_box(3, 3, 3, size=(1.0, 1.0, 1.0), jitter=0.1, seed=2)
+    rng = np.random.default_rng(2)
+    er, ur = O.material_tensors(rng.uniform(1, 5, mesh.n_tets) + 0j)
+    of = O.OracleNedelec2(mesh)
+    E64, B64 = O.tet_mass_stiffness_matrices(of, er, ur)
+    Eld, Bld = O.tet_mass_stiffness_matrices(of, er, ur, extended=True)
+    assert norm(E64 - Eld) / norm(Eld) < 1e-13 and norm(B64 - Bld) / norm(Bld) < 1e-13
+    # shrink to h = 0.01 and move far from the origin: fp64 loses digits, long double does not
+    far = O.OracleMesh(mesh.nodes * 0.03 + 0.5, mesh.tets)
+    near = O.OracleMesh(mesh.nodes * 0.03, mesh.tets)
+    Ef, _ = O.tet_mass_stiffness_matrices(O.OracleNedelec2(far), er, ur, extended=True)
+    En, _ = O.tet_mass_stiffness_matrices(O.OracleNedelec2(near), er, ur, extended=True)
+    assert norm(Ef - En) / norm(En) < 1e-13
