@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-face-mass", action="store_true")
+    ap.add_argument("--two-phase", action="store_true", help="time the per-tet-block fallback kernels")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 2 GiB device-to-device copy (reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the frequency-sharded "
@@ -82,7 +85,8 @@ def main():
     nu_d = torch.ones(mesh.n_tets, dtype=torch.complex128, device=dev)
     E = torch.empty(pat.nnz, dtype=torch.complex128, device=dev)
     B = torch.empty(pat.nnz, dtype=torch.complex128, device=dev)
-    flags = EXACT_FACE_MASS if args.exact_face_mass else 0
+    from emerge_amd.assembly import FORCE_TWO_PHASE
+    flags = (EXACT_FACE_MASS if args.exact_face_mass else 0) | (FORCE_TWO_PHASE if args.two_phase else 0)
 
     for _ in range(args.warmup):
         pat.assemble_iso_device(eps_d, nu_d, flags, E, B)
@@ -102,6 +106,12 @@ def main():
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     ms_per_step = wall / args.steps * 1e3
 
+    if args.copy_calibration:
+        src = torch.empty(2 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     nt, nv = mesh.n_tets, mesh.n_nodes
     alg_bytes = 2 * pat.nnz * 16 + nt * (16 + 32) + nv * 24
     achieved_gbs = alg_bytes / (kern_ms * 1e-3) / 1e9
@@ -126,7 +136,7 @@ def main():
                    "face_mass": "exact" if args.exact_face_mass else "reference"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_assemble<false>", "kernel_ms": kern_ms, "alg_bytes": alg_bytes,
+                     "kernel": "k_tet_blocks + k_gather_iso" if args.two_phase else "k_assemble_fused", "kernel_ms": kern_ms, "alg_bytes": alg_bytes,
                      "fp64_tflops": flops / (kern_ms * 1e-3) / 1e12,
                      "fp64_frac": flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
     }

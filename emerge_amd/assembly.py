@@ -16,6 +16,7 @@ from . import _lib
 from .mesh import Mesh3D
 
 EXACT_FACE_MASS = 1
+FORCE_TWO_PHASE = 2
 
 
 def is_isotropic(t: np.ndarray) -> bool:
@@ -96,11 +97,15 @@ class GPUPattern:
                 pass
             self._handle = None
 
-    def assemble(self, er: np.ndarray, ur: np.ndarray, exact_face_mass: bool = False, out=None):
-        """E, B values (nnz,) complex128 on the device for tensors er, ur (3,3,nt)."""
+    def assemble(self, er: np.ndarray, ur: np.ndarray, exact_face_mass: bool = False, out=None,
+                 two_phase: bool = False):
+        """E, B values (nnz,) complex128 on the device for tensors er, ur (3,3,nt).
+
+        two_phase selects the per-tet-block isotropic kernels instead of the
+        fused one (same results; used as a fallback and in tests)."""
         import torch
         L = _lib.load()
-        flags = EXACT_FACE_MASS if exact_face_mass else 0
+        flags = (EXACT_FACE_MASS if exact_face_mass else 0) | (FORCE_TWO_PHASE if two_phase else 0)
         if out is None:
             E = torch.empty(self.nnz, dtype=torch.complex128, device=self.device)
             B = torch.empty(self.nnz, dtype=torch.complex128, device=self.device)

@@ -29,6 +29,8 @@
 #include <cstdio>
 #include <cstring>
 #include <thread>
+#include <algorithm>
+#include <type_traits>
 
 #include <rocprim/rocprim.hpp>
 
@@ -54,9 +56,33 @@ struct emg_pattern {
     int32_t *pair_ptr = nullptr;  // nnz_ent + 1, into inc
     uint32_t *inc = nullptr;      // n_inc = 100*nt : (tet << 8) | (s << 4) | s'
     int32_t *blk_row0 = nullptr;  // nblk + 1, first row of each 256-entry block
+    double *tet_blocks = nullptr; // nt * TET_BLOCK_DOUBLES, two-phase fallback workspace
+    // fused path: per chunk of CHUNK adjacency entries, its sorted distinct tets
+    int64_t nchunk = 0;
+    int fused_ok = 0;
+    int32_t *chunk_tets = nullptr;   // nchunk * CHUNK_TMAX
+    int32_t *chunk_ntets = nullptr;  // nchunk
+    uint16_t *inc_local = nullptr;   // n_inc: (local tet << 8) | (s << 4) | s'
+    uint8_t *recipes = nullptr;      // 100 * RECIPE_BYTES
 };
 
 static constexpr int PAIR_BLOCK = 256;
+// Per-tet element record of the isotropic path: 55 upper (s <= s') 2x2 blocks
+// of D, 55 of F, and the 6 lower face-face blocks of F (the reference's face
+// mass is not symmetric there), each block (m,m') row-major.
+static constexpr int N_UPPER = 55;
+static constexpr int TET_BLOCK_DOUBLES = 4 * (N_UPPER + N_UPPER + 6);
+// Fused path: a workgroup owns CHUNK consecutive adjacency entries and stages
+// the Gram data of their (at most CHUNK_TMAX) distinct tets in LDS.
+static constexpr int CHUNK = 256;
+static constexpr int CHUNK_TMAX = 128;
+static constexpr int CHUNK_NMAX = 1024;  // incidences per chunk handled by the tet-list builder
+static constexpr int RECIPE_BYTES = 48;
+
+static void build_recipes(uint8_t *out);
+__global__ void __launch_bounds__(256) k_chunk_tets(const emg_pattern pat, int32_t *__restrict__ chunk_tets,
+                                                    int32_t *__restrict__ chunk_ntets, uint16_t *__restrict__ inc_local,
+                                                    int *__restrict__ overflow);
 
 // ------------------------------------------------------------------ symbolic
 __global__ void k_make_keys(const int32_t *__restrict__ tet_ent, int64_t nt, int bits,
@@ -218,6 +244,28 @@ extern "C" int emg_pattern_build(const int32_t *d_tet_ent, int64_t nt, int64_t n
     k_rows_from_keys<<<emg::grid_for(nnz_ent + 1, 256), 256, 0, st>>>(uniq, nnz_ent, bits, n_ent, pat->ent_ptr, pat->adj);
     k_block_rows<<<emg::grid_for(pat->nblk + 1, 256), 256, 0, st>>>(uniq, nnz_ent, bits, pat->nblk, n_ent, pat->blk_row0);
     TRY(hipGetLastError());
+    // fused-path chunk tables (chunks coincide with the PAIR_BLOCK row blocks)
+    static_assert(CHUNK == PAIR_BLOCK, "chunks reuse the row-block table");
+    pat->nchunk = pat->nblk;
+    {
+        int *d_over = nullptr;
+        TRY(hipMalloc(&pat->chunk_tets, (size_t)pat->nchunk * CHUNK_TMAX * sizeof(int32_t)));
+        TRY(hipMalloc(&pat->chunk_ntets, (size_t)pat->nchunk * sizeof(int32_t)));
+        TRY(hipMalloc(&pat->inc_local, (size_t)n_inc * sizeof(uint16_t)));
+        TRY(hipMalloc(&pat->recipes, 100 * RECIPE_BYTES));
+        TRY(hipMalloc(&d_over, sizeof(int)));
+        TRY(hipMemsetAsync(d_over, 0, sizeof(int), st));
+        uint8_t host_rec[100 * RECIPE_BYTES];
+        build_recipes(host_rec);
+        TRY(hipMemcpyAsync(pat->recipes, host_rec, sizeof(host_rec), hipMemcpyHostToDevice, st));
+        k_chunk_tets<<<(unsigned)pat->nchunk, 256, 0, st>>>(*pat, pat->chunk_tets, pat->chunk_ntets, pat->inc_local, d_over);
+        TRY(hipGetLastError());
+        int over = 0;
+        TRY(hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, st));
+        TRY(hipStreamSynchronize(st));
+        hfree(d_over);
+        pat->fused_ok = over ? 0 : 1;
+    }
     TRY(hipStreamSynchronize(st));
     hfree(uniq);
     hfree(counts);
@@ -254,6 +302,11 @@ extern "C" void emg_pattern_destroy(emg_pattern *pat) {
     hfree(pat->pair_ptr);
     hfree(pat->inc);
     hfree(pat->blk_row0);
+    hfree(pat->tet_blocks);
+    hfree(pat->chunk_tets);
+    hfree(pat->chunk_ntets);
+    hfree(pat->inc_local);
+    hfree(pat->recipes);
     delete pat;
 }
 
@@ -482,6 +535,435 @@ __global__ void __launch_bounds__(ABLK) k_assemble(const emg_pattern pat, const 
         }
 }
 
+// ------------------------------------------------------ isotropic fast path
+// Phase 1: one thread per canonical tet computes its real element blocks with
+// every index resolved at compile time (slots, vertex coincidences, moment
+// coefficients).  Phase 2: the row-owner gather scales them by the tet's
+// complex eps_r and 1/mu_r and writes each CSR value once.
+
+__host__ __device__ constexpr int slot_v(int s, int k) {
+    // canonical vertices (a, b, c) of local slot s; edges repeat b as c
+    return (s == 0) ? (k == 0 ? 0 : 1) : (s == 1) ? (k == 0 ? 0 : 2) : (s == 2) ? (k == 0 ? 0 : 3)
+         : (s == 3) ? (k == 0 ? 1 : 2) : (s == 4) ? (k == 0 ? 1 : 3) : (s == 5) ? (k == 0 ? 2 : 3)
+         : (s == 6) ? k : (s == 7) ? (k == 0 ? 0 : k + 1) : (s == 8) ? (k == 2 ? 3 : k) : k + 1;
+}
+__host__ __device__ constexpr int edge_index(int u, int v) {  // u < v
+    return u == 0 ? v - 1 : (u == 1 ? v + 1 : 5);
+}
+__host__ __device__ constexpr int fact(int n) { return n <= 1 ? 1 : n * fact(n - 1); }
+__host__ __device__ constexpr int moment(int w, int x, int y, int z) {
+    int r = 1;
+    for (int k = 0; k < 4; k++) r *= fact((w == k) + (x == k) + (y == k) + (z == k));
+    return r;
+}
+__host__ __device__ constexpr int upper_index(int a, int b) { return a * 10 - a * (a - 1) / 2 + (b - a); }
+
+template <int B, int E, typename Fn>
+__device__ __forceinline__ void static_for(Fn &&fn) {
+    if constexpr (B < E) {
+        fn(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(fn);
+    }
+}
+
+struct TetGeo {
+    double G[4][4];  // g_a . g_b / (5040 * 6V)
+    double H[6][6];  // X_k . X_l / (120 * (6V)^3), X = g_a x g_b over canonical edges
+    double L[6];     // canonical edge lengths
+};
+
+// Real 2x2 (mode, mode') blocks of D and F for local slots (S, S2).  fq holds
+// F with the reference's face-face coefficient, fx with the exact one.
+template <int S, int S2>
+__device__ __forceinline__ void iso_block(const TetGeo &g, double d[2][2], double fq[2][2], double fx[2][2]) {
+    constexpr bool fi = S >= 6, fj = S2 >= 6;
+    constexpr int ia = slot_v(S, 0), ib = slot_v(S, 1), ic = slot_v(S, 2);
+    constexpr int ja = slot_v(S2, 0), jb = slot_v(S2, 1), jc = slot_v(S2, 2);
+    // curl terms: cross index, paired vertex, coefficient per mode
+    constexpr int ki[3] = {edge_index(ia, ib), edge_index(ia, fi ? ic : ib), fi ? edge_index(ib, ic) : 0};
+    constexpr int kj[3] = {edge_index(ja, jb), edge_index(ja, fj ? jc : jb), fj ? edge_index(jb, jc) : 0};
+    constexpr int tvi[3] = {fi ? ic : ia, ib, fi ? ia : ib};
+    constexpr int tvj[3] = {fj ? jc : ja, jb, fj ? ja : jb};
+    constexpr int cfi[2][3] = {{fi ? -1 : -3, fi ? -2 : 0, fi ? -1 : 0}, {fi ? -2 : 0, fi ? -1 : -3, fi ? 1 : 0}};
+    constexpr int cfj[2][3] = {{fj ? -1 : -3, fj ? -2 : 0, fj ? -1 : 0}, {fj ? -2 : 0, fj ? -1 : -3, fj ? 1 : 0}};
+    const double si[2] = {fi ? -g.L[edge_index(ia, ic)] : g.L[edge_index(ia, ib)], g.L[edge_index(ia, ib)]};
+    const double sj[2] = {fj ? -g.L[edge_index(ja, jc)] : g.L[edge_index(ja, jb)], g.L[edge_index(ja, jb)]};
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+#pragma unroll
+        for (int m2 = 0; m2 < 2; m2++) {
+            double dv = 0.0;
+#pragma unroll
+            for (int x = 0; x < 3; x++)
+#pragma unroll
+                for (int y = 0; y < 3; y++) {
+                    const int c = cfi[m][x] * cfj[m2][y] * ((tvi[x] == tvj[y]) ? 2 : 1);
+                    if (c != 0) dv += (double)c * g.H[ki[x]][kj[y]];
+                }
+            // basis (p, q, r): edge m1 (a,b,a), edge m2 (b,b,a), face m1 (b,c,a), face m2 (c,b,a)
+            const int p = (m == 0) ? (fi ? ib : ia) : (fi ? ic : ib), q = (m == 0 && fi) ? ic : ib, r = ia;
+            const int p2 = (m2 == 0) ? (fj ? jb : ja) : (fj ? jc : jb), q2 = (m2 == 0 && fj) ? jc : jb, r2 = ja;
+            const double base = moment(p, q, p2, q2) * g.G[r][r2] - moment(p, r, p2, q2) * g.G[q][r2] +
+                                moment(p, r, p2, r2) * g.G[q][q2];
+            const int c_exact = moment(p, q, p2, r2);
+            const int c_ref = (fi && fj && m2 == 0) ? moment(ib, r2, p2, q2) : c_exact;
+            const double sij = si[m] * sj[m2];
+            d[m][m2] = sij * dv;
+            fq[m][m2] = sij * (base - c_ref * g.G[r][q2]);
+            fx[m][m2] = sij * (base - c_exact * g.G[r][q2]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_tet_blocks(int64_t nt, const double *__restrict__ nodes,
+                                                   const int32_t *__restrict__ tets, int exact_face_mass,
+                                                   double *__restrict__ rec) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
+    const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+    V3 P[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) P[i] = {nodes[3 * (int64_t)vid[i]], nodes[3 * (int64_t)vid[i] + 1], nodes[3 * (int64_t)vid[i] + 2]};
+    const V3 e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]), e3 = sub(P[3], P[0]);
+    V3 gr[4];
+    gr[1] = crs(e2, e3);
+    gr[2] = crs(e3, e1);
+    gr[3] = crs(e1, e2);
+    gr[0] = {-(gr[1].x + gr[2].x + gr[3].x), -(gr[1].y + gr[2].y + gr[3].y), -(gr[1].z + gr[2].z + gr[3].z)};
+    const double V6 = fabs(dot(e1, gr[1]));
+    const double fscale = 1.0 / (5040.0 * V6);
+    const double dscale = 1.0 / (120.0 * V6 * V6 * V6);
+    TetGeo g;
+    constexpr int EU[6] = {0, 0, 0, 1, 1, 2}, EV[6] = {1, 2, 3, 2, 3, 3};
+    V3 X[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        X[k] = crs(gr[EU[k]], gr[EV[k]]);
+        g.L[k] = nrm(sub(P[EV[k]], P[EU[k]]));
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = a; b < 4; b++) g.G[a][b] = g.G[b][a] = dot(gr[a], gr[b]) * fscale;
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+#pragma unroll
+        for (int l = k; l < 6; l++) g.H[k][l] = g.H[l][k] = dot(X[k], X[l]) * dscale;
+    double *out = rec + t * TET_BLOCK_DOUBLES;
+    const bool exact = exact_face_mass != 0;
+    static_for<0, 10>([&](auto si_) {
+        constexpr int S = decltype(si_)::value;
+        static_for<S, 10>([&](auto sj_) {
+            constexpr int S2 = decltype(sj_)::value;
+            double d[2][2], fq[2][2], fx[2][2];
+            iso_block<S, S2>(g, d, fq, fx);
+            constexpr int u = upper_index(S, S2);
+            double2 *od = reinterpret_cast<double2 *>(out + 4 * u);
+            double2 *of = reinterpret_cast<double2 *>(out + 4 * (N_UPPER + u));
+            od[0] = make_double2(d[0][0], d[0][1]);
+            od[1] = make_double2(d[1][0], d[1][1]);
+            of[0] = make_double2(exact ? fx[0][0] : fq[0][0], exact ? fx[0][1] : fq[0][1]);
+            of[1] = make_double2(exact ? fx[1][0] : fq[1][0], exact ? fx[1][1] : fq[1][1]);
+        });
+    });
+    // lower face-face blocks of F (s > s' >= 6)
+    static_for<7, 10>([&](auto si_) {
+        constexpr int S = decltype(si_)::value;
+        static_for<6, S>([&](auto sj_) {
+            constexpr int S2 = decltype(sj_)::value;
+            double d[2][2], fq[2][2], fx[2][2];
+            iso_block<S, S2>(g, d, fq, fx);
+            constexpr int x = (S - 6) * (S - 7) / 2 + (S2 - 6);
+            double2 *of = reinterpret_cast<double2 *>(out + 4 * (2 * N_UPPER + x));
+            of[0] = make_double2(exact ? fx[0][0] : fq[0][0], exact ? fx[0][1] : fq[0][1]);
+            of[1] = make_double2(exact ? fx[1][0] : fq[1][0], exact ? fx[1][1] : fq[1][1]);
+        });
+    });
+}
+
+__global__ void __launch_bounds__(256) k_gather_iso(const emg_pattern pat, const double *__restrict__ rec,
+                                                    const cplx *__restrict__ eps, const cplx *__restrict__ nu,
+                                                    cplx *__restrict__ E, cplx *__restrict__ B) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= pat.nnz_ent) return;
+    const int e = find_row(pat.ent_ptr, pat.blk_row0, k);
+    const int32_t p0 = pat.ent_ptr[e];
+    const int32_t deg = pat.ent_ptr[e + 1] - p0;
+    cplx aD[2][2], aF[2][2];
+    const int32_t q0 = pat.pair_ptr[k], q1 = pat.pair_ptr[k + 1];
+    for (int32_t q = q0; q < q1; q++) {
+        const uint32_t v = pat.inc[q];
+        const int t = (int)(v >> 8);
+        const int s = (int)((v >> 4) & 15u), s2 = (int)(v & 15u);
+        const double *r = rec + (int64_t)t * TET_BLOCK_DOUBLES;
+        const bool up = s <= s2;
+        const int a = up ? s : s2, b = up ? s2 : s;
+        const int u = upper_index(a, b);
+        const double2 d0 = *reinterpret_cast<const double2 *>(r + 4 * u);
+        const double2 d1 = *reinterpret_cast<const double2 *>(r + 4 * u + 2);
+        const bool lowff = !up && s2 >= 6;   // stored lower face-face block of F
+        const int fo = lowff ? 4 * (2 * N_UPPER + (s - 6) * (s - 7) / 2 + (s2 - 6)) : 4 * (N_UPPER + u);
+        const double2 f0 = *reinterpret_cast<const double2 *>(r + fo);
+        const double2 f1 = *reinterpret_cast<const double2 *>(r + fo + 2);
+        const bool tf = !up && !lowff;       // F read from the transposed block
+        const double D00 = d0.x, D11 = d1.y, D01 = up ? d0.y : d1.x, D10 = up ? d1.x : d0.y;
+        const double F00 = f0.x, F11 = f1.y, F01 = tf ? f1.x : f0.y, F10 = tf ? f0.y : f1.x;
+        const cplx ct = eps[t], nt_ = nu[t];
+        aD[0][0] += nt_ * D00; aD[0][1] += nt_ * D01; aD[1][0] += nt_ * D10; aD[1][1] += nt_ * D11;
+        aF[0][0] += ct * F00;  aF[0][1] += ct * F01;  aF[1][0] += ct * F10;  aF[1][1] += ct * F11;
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int m2 = 0; m2 < 2; m2++) {
+            const int64_t idx = (int64_t)m * 2 * pat.nnz_ent + p0 + (int64_t)m2 * deg + k;
+            E[idx] = aD[m][m2];
+            B[idx] = aF[m][m2];
+        }
+}
+
+// ------------------------------------------------------------ fused path
+// Recipe of an ordered slot pair (s, s'), RECIPE_BYTES bytes:
+//   [0..8]   H index (symmetric 6x6 over canonical edges) of block (x, y)
+//   [9..17]  G index (symmetric 4x4 over vertices) of block (x, y)
+//   [18..19] bit x*3+y: paired curl vertices coincide (integral weight 2)
+//   [20..35] per (m, m') the moments of the 4 mass terms (exact)
+//   [36..39] per (m, m') the second mass moment as the reference indexes it
+//   [40..43] length index of i mode 0/1 and j mode 0/1
+//   [44]     bit0 row is a face, bit1 column is a face
+__host__ __device__ constexpr int sym4(int a, int b) { return a <= b ? a * 4 - a * (a - 1) / 2 + (b - a) : sym4(b, a); }
+__host__ __device__ constexpr int sym6(int a, int b) { return a <= b ? a * 6 - a * (a - 1) / 2 + (b - a) : sym6(b, a); }
+
+static void build_recipes(uint8_t *out) {
+    for (int s = 0; s < 10; s++)
+        for (int s2 = 0; s2 < 10; s2++) {
+            uint8_t *R = out + (s * 10 + s2) * RECIPE_BYTES;
+            for (int i = 0; i < RECIPE_BYTES; i++) R[i] = 0;
+            const bool fi = s >= 6, fj = s2 >= 6;
+            const int vi[3] = {slot_v(s, 0), slot_v(s, 1), slot_v(s, 2)};
+            const int vj[3] = {slot_v(s2, 0), slot_v(s2, 1), slot_v(s2, 2)};
+            const int ki[3] = {edge_index(vi[0], vi[1]), edge_index(vi[0], fi ? vi[2] : vi[1]), fi ? edge_index(vi[1], vi[2]) : 0};
+            const int kj[3] = {edge_index(vj[0], vj[1]), edge_index(vj[0], fj ? vj[2] : vj[1]), fj ? edge_index(vj[1], vj[2]) : 0};
+            const int tvi[3] = {fi ? vi[2] : vi[0], vi[1], fi ? vi[0] : vi[1]};
+            const int tvj[3] = {fj ? vj[2] : vj[0], vj[1], fj ? vj[0] : vj[1]};
+            int w = 0;
+            for (int x = 0; x < 3; x++)
+                for (int y = 0; y < 3; y++) {
+                    R[x * 3 + y] = (uint8_t)sym6(ki[x], kj[y]);
+                    R[9 + x * 3 + y] = (uint8_t)sym4(vi[x], vj[y]);
+                    if (tvi[x] == tvj[y]) w |= 1 << (x * 3 + y);
+                }
+            R[18] = (uint8_t)(w & 255);
+            R[19] = (uint8_t)(w >> 8);
+            for (int m = 0; m < 2; m++)
+                for (int m2 = 0; m2 < 2; m2++) {
+                    const int p = (m == 0) ? (fi ? vi[1] : vi[0]) : (fi ? vi[2] : vi[1]);
+                    const int q = (m == 0 && fi) ? vi[2] : vi[1], r = vi[0];
+                    const int p2 = (m2 == 0) ? (fj ? vj[1] : vj[0]) : (fj ? vj[2] : vj[1]);
+                    const int q2 = (m2 == 0 && fj) ? vj[2] : vj[1], r2 = vj[0];
+                    uint8_t *c = R + 20 + 4 * (m * 2 + m2);
+                    c[0] = (uint8_t)moment(p, q, p2, q2);
+                    c[1] = (uint8_t)moment(p, q, p2, r2);
+                    c[2] = (uint8_t)moment(p, r, p2, q2);
+                    c[3] = (uint8_t)moment(p, r, p2, r2);
+                    R[36 + m * 2 + m2] = (fi && fj && m2 == 0) ? (uint8_t)moment(vi[1], r2, p2, q2) : c[1];
+                }
+            R[40] = (uint8_t)(fi ? edge_index(vi[0], vi[2]) : edge_index(vi[0], vi[1]));
+            R[41] = (uint8_t)edge_index(vi[0], vi[1]);
+            R[42] = (uint8_t)(fj ? edge_index(vj[0], vj[2]) : edge_index(vj[0], vj[1]));
+            R[43] = (uint8_t)edge_index(vj[0], vj[1]);
+            R[44] = (uint8_t)((fi ? 1 : 0) | (fj ? 2 : 0));
+        }
+}
+
+// Distinct tets of every chunk, and chunk-local incidence codes.
+__global__ void __launch_bounds__(256) k_chunk_tets(const emg_pattern pat, int32_t *__restrict__ chunk_tets,
+                                                    int32_t *__restrict__ chunk_ntets, uint16_t *__restrict__ inc_local,
+                                                    int *__restrict__ overflow) {
+    __shared__ uint32_t key[CHUNK_NMAX];
+    __shared__ uint32_t uq[CHUNK_NMAX];
+    __shared__ int nuniq;
+    const int64_t c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int64_t k0 = c * CHUNK, k1 = (k0 + CHUNK < pat.nnz_ent) ? k0 + CHUNK : pat.nnz_ent;
+    const int32_t q0 = pat.pair_ptr[k0], q1 = pat.pair_ptr[k1];
+    const int n = q1 - q0;
+    if (n > CHUNK_NMAX) {
+        if (tid == 0) { atomicOr(overflow, 1); chunk_ntets[c] = -1; }
+        return;
+    }
+    for (int i = tid; i < CHUNK_NMAX; i += 256) key[i] = (i < n) ? (pat.inc[q0 + i] >> 8) : 0xFFFFFFFFu;
+    __syncthreads();
+    for (int len = 2; len <= CHUNK_NMAX; len <<= 1)
+        for (int stride = len >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < CHUNK_NMAX; i += 256) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool asc = (i & len) == 0;
+                    const uint32_t a = key[i], b = key[j];
+                    if ((a > b) == asc) { key[i] = b; key[j] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    if (tid == 0) {
+        int m = 0;
+        for (int i = 0; i < n; i++)
+            if (i == 0 || key[i] != key[i - 1]) uq[m++] = key[i];
+        nuniq = m;
+    }
+    __syncthreads();
+    const int m = nuniq;
+    if (m > CHUNK_TMAX) {
+        if (tid == 0) { atomicOr(overflow, 1); chunk_ntets[c] = -1; }
+        return;
+    }
+    for (int i = tid; i < m; i += 256) chunk_tets[c * CHUNK_TMAX + i] = (int32_t)uq[i];
+    if (tid == 0) chunk_ntets[c] = m;
+    for (int i = tid; i < n; i += 256) {
+        const uint32_t v = pat.inc[q0 + i];
+        const uint32_t t = v >> 8;
+        int lo = 0, hi = m - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (uq[mid] < t) lo = mid + 1; else hi = mid;
+        }
+        inc_local[q0 + i] = (uint16_t)((lo << 8) | (v & 255u));
+    }
+}
+
+static constexpr int TD_FIELDS = 41;  // H(21) G(10) L(6) eps(2) nu(2)
+
+__device__ __forceinline__ uint32_t rbyte(const uint32_t *w, int i) { return (w[i >> 2] >> ((i & 3) * 8)) & 255u; }
+
+__global__ void __launch_bounds__(256) k_assemble_fused(const emg_pattern pat, const double *__restrict__ nodes,
+                                                        const int32_t *__restrict__ tets, const cplx *__restrict__ eps,
+                                                        const cplx *__restrict__ nu, int exact_face_mass,
+                                                        cplx *__restrict__ E, cplx *__restrict__ B) {
+    __shared__ uint32_t rcp[100 * RECIPE_BYTES / 4];
+    __shared__ double td[TD_FIELDS * CHUNK_TMAX];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 100 * RECIPE_BYTES / 4; i += 256) rcp[i] = reinterpret_cast<const uint32_t *>(pat.recipes)[i];
+    for (int64_t c = blockIdx.x; c < pat.nchunk; c += gridDim.x) {
+        __syncthreads();
+        const int nT = pat.chunk_ntets[c];
+        if (tid < nT) {
+            const int t = pat.chunk_tets[c * CHUNK_TMAX + tid];
+            const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
+            const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+            V3 P[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                P[i] = {nodes[3 * (int64_t)vid[i]], nodes[3 * (int64_t)vid[i] + 1], nodes[3 * (int64_t)vid[i] + 2]};
+            const V3 e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]), e3 = sub(P[3], P[0]);
+            V3 gr[4];
+            gr[1] = crs(e2, e3);
+            gr[2] = crs(e3, e1);
+            gr[3] = crs(e1, e2);
+            gr[0] = {-(gr[1].x + gr[2].x + gr[3].x), -(gr[1].y + gr[2].y + gr[3].y), -(gr[1].z + gr[2].z + gr[3].z)};
+            const double V6 = fabs(dot(e1, gr[1]));
+            const double fscale = 1.0 / (5040.0 * V6);
+            const double dscale = 1.0 / (120.0 * V6 * V6 * V6);
+            constexpr int EU[6] = {0, 0, 0, 1, 1, 2}, EV[6] = {1, 2, 3, 2, 3, 3};
+            V3 X[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                X[k] = crs(gr[EU[k]], gr[EV[k]]);
+                td[(31 + k) * CHUNK_TMAX + tid] = nrm(sub(P[EV[k]], P[EU[k]]));
+            }
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+#pragma unroll
+                for (int l = k; l < 6; l++) td[sym6(k, l) * CHUNK_TMAX + tid] = dot(X[k], X[l]) * dscale;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = a; b < 4; b++) td[(21 + sym4(a, b)) * CHUNK_TMAX + tid] = dot(gr[a], gr[b]) * fscale;
+            const cplx ev = eps[t], nv = nu[t];
+            td[37 * CHUNK_TMAX + tid] = ev.re;
+            td[38 * CHUNK_TMAX + tid] = ev.im;
+            td[39 * CHUNK_TMAX + tid] = nv.re;
+            td[40 * CHUNK_TMAX + tid] = nv.im;
+        }
+        __syncthreads();
+        const int64_t k = c * CHUNK + tid;
+        if (k >= pat.nnz_ent) continue;
+        const int e = find_row(pat.ent_ptr, pat.blk_row0, k);
+        const int32_t p0 = pat.ent_ptr[e];
+        const int32_t deg = pat.ent_ptr[e + 1] - p0;
+        cplx aD[2][2], aF[2][2];
+        const int32_t q0 = pat.pair_ptr[k], q1 = pat.pair_ptr[k + 1];
+        for (int32_t q = q0; q < q1; q++) {
+            const uint32_t v = pat.inc_local[q];
+            const int tl = (int)(v >> 8);
+            const int sp = (int)(((v >> 4) & 15u) * 10 + (v & 15u));
+            uint32_t w[12];
+            const uint4 *R4 = reinterpret_cast<const uint4 *>(rcp + sp * (RECIPE_BYTES / 4));
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const uint4 u = R4[i];
+                w[4 * i] = u.x; w[4 * i + 1] = u.y; w[4 * i + 2] = u.z; w[4 * i + 3] = u.w;
+            }
+            const double *T = td + tl;
+            const uint32_t wm = rbyte(w, 18) | (rbyte(w, 19) << 8);
+            double h[3][3], g[3][3];
+#pragma unroll
+            for (int x = 0; x < 3; x++)
+#pragma unroll
+                for (int y = 0; y < 3; y++) {
+                    const double hv = T[rbyte(w, x * 3 + y) * CHUNK_TMAX];
+                    h[x][y] = ((wm >> (x * 3 + y)) & 1u) ? 2.0 * hv : hv;
+                    g[x][y] = T[(21 + rbyte(w, 9 + x * 3 + y)) * CHUNK_TMAX];
+                }
+            const uint32_t fl = rbyte(w, 44);
+            const bool fi = fl & 1u, fj = fl & 2u;
+            const double cfi[2][3] = {{fi ? -1.0 : -3.0, fi ? -2.0 : 0.0, fi ? -1.0 : 0.0},
+                                      {fi ? -2.0 : 0.0, fi ? -1.0 : -3.0, fi ? 1.0 : 0.0}};
+            const double cfj[2][3] = {{fj ? -1.0 : -3.0, fj ? -2.0 : 0.0, fj ? -1.0 : 0.0},
+                                      {fj ? -2.0 : 0.0, fj ? -1.0 : -3.0, fj ? 1.0 : 0.0}};
+            const double si[2] = {(fi ? -1.0 : 1.0) * T[(31 + rbyte(w, 40)) * CHUNK_TMAX], T[(31 + rbyte(w, 41)) * CHUNK_TMAX]};
+            const double sj[2] = {(fj ? -1.0 : 1.0) * T[(31 + rbyte(w, 42)) * CHUNK_TMAX], T[(31 + rbyte(w, 43)) * CHUNK_TMAX]};
+            const cplx ct(T[37 * CHUNK_TMAX], T[38 * CHUNK_TMAX]), nt_(T[39 * CHUNK_TMAX], T[40 * CHUNK_TMAX]);
+            double hc[3][2];
+#pragma unroll
+            for (int x = 0; x < 3; x++)
+#pragma unroll
+                for (int m2 = 0; m2 < 2; m2++)
+                    hc[x][m2] = h[x][0] * cfj[m2][0] + h[x][1] * cfj[m2][1] + h[x][2] * cfj[m2][2];
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const bool qc = (m == 0) && fi;
+                const double g_q0 = qc ? g[2][0] : g[1][0];
+#pragma unroll
+                for (int m2 = 0; m2 < 2; m2++) {
+                    const bool qc2 = (m2 == 0) && fj;
+                    const double g_0q = qc2 ? g[0][2] : g[0][1];
+                    const double g_qq = qc ? (qc2 ? g[2][2] : g[2][1]) : (qc2 ? g[1][2] : g[1][1]);
+                    const int o = 20 + 4 * (m * 2 + m2);
+                    const double c1 = (double)(exact_face_mass ? rbyte(w, o + 1) : rbyte(w, 36 + m * 2 + m2));
+                    const double f = (double)rbyte(w, o) * g[0][0] - c1 * g_0q - (double)rbyte(w, o + 2) * g_q0 +
+                                     (double)rbyte(w, o + 3) * g_qq;
+                    const double d = cfi[m][0] * hc[0][m2] + cfi[m][1] * hc[1][m2] + cfi[m][2] * hc[2][m2];
+                    const double sij = si[m] * sj[m2];
+                    aD[m][m2] += nt_ * (sij * d);
+                    aF[m][m2] += ct * (sij * f);
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int m2 = 0; m2 < 2; m2++) {
+                const int64_t idx = (int64_t)m * 2 * pat.nnz_ent + p0 + (int64_t)m2 * deg + k;
+                E[idx] = aD[m][m2];
+                B[idx] = aF[m][m2];
+            }
+    }
+}
+
+static int g_num_cus = 0;
+
 template <bool TENSOR>
 static int assemble(const emg_pattern *pat, const double *d_nodes, const int32_t *d_tets, const double *d_m,
                     const double *d_s, int flags, double *d_E, double *d_B, void *stream) {
@@ -497,7 +979,33 @@ static int assemble(const emg_pattern *pat, const double *d_nodes, const int32_t
 
 extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, const int32_t *d_tets, const double *d_eps,
                                 const double *d_nu, int flags, double *d_E, double *d_B, void *stream) {
-    return assemble<false>(pat, d_nodes, d_tets, d_eps, d_nu, flags, d_E, d_B, stream);
+    EMG_CHECK(pat && d_nodes && d_tets && d_eps && d_nu && d_E && d_B, "emg_assemble_iso: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    emg_pattern *mp = const_cast<emg_pattern *>(pat);
+    if (pat->fused_ok && !(flags & EMG_FORCE_TWO_PHASE)) {
+        if (g_num_cus == 0) {
+            int dev = 0;
+            EMG_HIP(hipGetDevice(&dev));
+            EMG_HIP(hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+        }
+        const int64_t grid = std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * 8);
+        k_assemble_fused<<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, reinterpret_cast<const cplx *>(d_eps),
+                                                         reinterpret_cast<const cplx *>(d_nu),
+                                                         (flags & EMG_EXACT_FACE_MASS) ? 1 : 0,
+                                                         reinterpret_cast<cplx *>(d_E), reinterpret_cast<cplx *>(d_B));
+        EMG_LAUNCH_CHECK();
+        return 0;
+    }
+    if (!mp->tet_blocks) EMG_HIP(hipMalloc(&mp->tet_blocks, (size_t)pat->nt * TET_BLOCK_DOUBLES * sizeof(double)));
+    k_tet_blocks<<<emg::grid_for(pat->nt, 64), 64, 0, st>>>(pat->nt, d_nodes, d_tets,
+                                                            (flags & EMG_EXACT_FACE_MASS) ? 1 : 0, mp->tet_blocks);
+    k_gather_iso<<<emg::grid_for(pat->nnz_ent, 256), 256, 0, st>>>(*pat, mp->tet_blocks,
+                                                                   reinterpret_cast<const cplx *>(d_eps),
+                                                                   reinterpret_cast<const cplx *>(d_nu),
+                                                                   reinterpret_cast<cplx *>(d_E),
+                                                                   reinterpret_cast<cplx *>(d_B));
+    EMG_LAUNCH_CHECK();
+    return 0;
 }
 
 extern "C" int emg_assemble_tensor(const emg_pattern *pat, const double *d_nodes, const int32_t *d_tets,

@@ -53,6 +53,8 @@ void emg_pattern_destroy(emg_pattern *pat);
 /* Flags for emg_assemble_*. */
 #define EMG_EXACT_FACE_MASS 1 /* use the exact face-face mass integral instead of
                                  the reference's coefficient (fem/mth/tet.py:1036) */
+#define EMG_FORCE_TWO_PHASE 2 /* isotropic assembly through per-tet element blocks in
+                                 HBM instead of the fused LDS kernel (same results) */
 
 /* Numeric assembly of the curl-curl matrix E (Dmat, with 1/mu) and the mass
  * matrix B (Fmat, with eps) into the pattern's CSR value arrays, complex128.

@@ -19,9 +19,9 @@ def _materials(nt, rng, tensor=False):
     return er, ur
 
 
-def _check(mesh, er, ur, exact=False, order_seed=11):
+def _check(mesh, er, ur, exact=False, order_seed=11, two_phase=False):
     pat = GPUPattern(mesh)
-    E, B = pat.assemble(er, ur, exact_face_mass=exact)
+    E, B = pat.assemble(er, ur, exact_face_mass=exact, two_phase=two_phase)
     Eg, Bg = pat.to_scipy(E), pat.to_scipy(B)
     om = O.OracleMesh(mesh.nodes, mesh.tets, order_seed=order_seed)
     of = O.OracleNedelec2(om)
@@ -30,12 +30,13 @@ def _check(mesh, er, ur, exact=False, order_seed=11):
     return Eg, Bg, Eo, Bo
 
 
+@pytest.mark.parametrize("two_phase", [False, True])
 @pytest.mark.parametrize("shape,seed", [((1, 1, 1), 0), ((3, 2, 2), 1), ((4, 3, 5), 2)])
-def test_iso_matches_oracle(cuda, shape, seed):
-    print(f"iso shape={shape} seed={seed}", flush=True)
+def test_iso_matches_oracle(cuda, shape, seed, two_phase):
+    print(f"iso shape={shape} seed={seed} two_phase={two_phase}", flush=True)
     mesh = meshgen.structured_box(*shape, size=(1.0, 0.7, 1.3), jitter=0.1, seed=seed)
     er, ur = _materials(mesh.n_tets, np.random.default_rng(seed))
-    Eg, Bg, Eo, Bo = _check(mesh, er, ur)
+    Eg, Bg, Eo, Bo = _check(mesh, er, ur, two_phase=two_phase)
     assert rel_fro(Eg, Eo) < 1e-12 and rel_max(Eg, Eo) < 1e-12
     assert rel_fro(Bg, Bo) < 1e-12 and rel_max(Bg, Bo) < 1e-12
     # CSR structure: sorted unique columns, pattern contains the oracle's
@@ -52,13 +53,14 @@ def test_tensor_matches_oracle(cuda):
     assert rel_fro(Bg, Bo) < 1e-12 and rel_max(Bg, Bo) < 1e-12
 
 
-def test_exact_face_mass_is_symmetric(cuda):
-    print("exact face mass", flush=True)
+@pytest.mark.parametrize("two_phase", [False, True])
+def test_exact_face_mass_is_symmetric(cuda, two_phase):
+    print(f"exact face mass two_phase={two_phase}", flush=True)
     mesh = meshgen.structured_box(2, 2, 2, jitter=0.1, seed=6)
     er, ur = _materials(mesh.n_tets, np.random.default_rng(6))
     pat = GPUPattern(mesh)
-    _, B0 = pat.assemble(er, ur)
-    _, B1 = pat.assemble(er, ur, exact_face_mass=True)
+    _, B0 = pat.assemble(er, ur, two_phase=two_phase)
+    _, B1 = pat.assemble(er, ur, exact_face_mass=True, two_phase=two_phase)
     B0, B1 = pat.to_scipy(B0), pat.to_scipy(B1)
     assert abs(B1 - B1.T).max() < 1e-14 * abs(B1).max()
     assert abs(B0 - B0.T).max() > 1e-6 * abs(B0).max()     # the reference's is not
