@@ -26,7 +26,22 @@ SIGNATURES: dict[str, tuple] = {
     "emg_pattern_destroy": (None, [P]),
     "emg_assemble_iso": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, P, P, P]),
     "emg_assemble_tensor": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, P, P, P]),
+    "emg_solver_create": (ctypes.c_int, [I64, I64, P, P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]),
+    "emg_solver_set_matrix": (ctypes.c_int, [P, P, P, P]),
+    "emg_solver_solve": (ctypes.c_int, [P, P, P, F64, ctypes.c_int, ctypes.c_int, P, P]),
+    "emg_solver_destroy": (None, [P]),
+    "emg_affine_combine": (ctypes.c_int, [I64, P, P, F64, P, P]),
+    "emg_csr_find": (ctypes.c_int, [I64, P, P, P, P, P, P]),
+    "emg_scatter_add": (ctypes.c_int, [I64, P, P, F64, F64, P, P]),
+    "emg_robin_blocks": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P, P]),
+    "emg_interp_points": (ctypes.c_int, [I64, P, P, P, P, P, I64, ctypes.c_int, P, P, P]),
 }
+
+
+class SolveInfo(ctypes.Structure):
+    """emg_solve_info (include/emerge_amd.h)."""
+    _fields_ = [("iterations", ctypes.c_int), ("refinements", ctypes.c_int), ("converged", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("residual", ctypes.c_double)]
 
 _lib = None
 

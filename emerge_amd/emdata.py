@@ -1,0 +1,144 @@
+"""Result containers with the reference's access pattern
+(fem/physics/edm/emdata.py:34-333, fem/dataset.py:61-83):
+
+    freqs, S21 = data.ax('freq').S(2, 1)
+    S21_at_3 = data.item(3).S(2, 1)
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+class Sparam:
+    """Port-labelled square S matrix (emdata.py:34-88)."""
+
+    def __init__(self, port_nrs):
+        self.map = {label: i for i, label in enumerate(port_nrs)}
+        self.arry = np.zeros((len(port_nrs), len(port_nrs)), dtype=np.complex128)
+
+    def get(self, p1, p2):
+        try:
+            return self.arry[self.map[p1], self.map[p2]]
+        except KeyError as e:
+            raise KeyError(f"Port/mode {e.args[0]!r} not found in mapping") from None
+
+    def set(self, p1, p2, value):
+        self.arry[self.map[p1], self.map[p2]] = value
+
+    __call__ = get
+
+    def __getitem__(self, key):
+        return self.get(*key)
+
+    def __setitem__(self, key, value):
+        self.set(key[0], key[1], value)
+
+
+@dataclass
+class PortProperties:
+    """emdata.py:90-97."""
+    port_number: int | None = None
+    k0: float | None = None
+    beta: complex | None = None
+    Z0: float | None = None
+    Pout: float | None = None
+    mode_number: int = 1
+
+
+@dataclass
+class SolveRecord:
+    """Per-solve diagnostics (no counterpart in the reference, which discards them)."""
+    port: int
+    iterations: int
+    refinements: int
+    residual: float
+    converged: bool
+    solver: str
+
+
+class EMDataSet:
+    """One frequency point (emdata.py:99-203)."""
+
+    def __init__(self, **kw):
+        self.freq: float | None = None
+        self.k0: float | None = None
+        self.er = None
+        self.ur = None
+        self.Sp: Sparam | None = None
+        self._fields: dict[int, np.ndarray] = {}
+        self.port_modes: list[PortProperties] = []
+        self.excitation: dict[int, complex] = {}
+        self.solves: list[SolveRecord] = []
+        self.mode = None
+        self.beta = None
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+    def init_sp(self, port_numbers):
+        self.Sp = Sparam(port_numbers)
+
+    def add_port_properties(self, port_number, mode_number, k0, beta, Z0, Pout):
+        self.port_modes.append(PortProperties(port_number, k0, beta, Z0, Pout, mode_number))
+
+    def write_S(self, i1, i2, value):
+        self.Sp[i1, i2] = value
+
+    def S(self, i1, i2):
+        return self.Sp(i1, i2)
+
+    def set_field_vector(self):
+        self.excitation = {k: 0.0 for k in self._fields}
+        if self.port_modes:
+            self.excitation[self.port_modes[0].port_number] = 1.0 + 0j
+
+    @property
+    def _field(self):
+        return sum(self.excitation[m.port_number] * self._fields[m.port_number] for m in self.port_modes
+                   if m.port_number in self._fields)
+
+
+class _Proxy:
+    """Collects a property or method result of every dataset along an axis (emdata.py:206-243)."""
+
+    def __init__(self, axis: str, dss):
+        self._axis = axis
+        self._dss = dss
+
+    def __getattr__(self, name):
+        dss, axis = self._dss, self._axis
+        if callable(getattr(dss[0], name)):
+            def wrapped(*a, **k):
+                return (np.array([getattr(d, axis) for d in dss]),
+                        np.array([getattr(d, name)(*a, **k) for d in dss]))
+            return wrapped
+        return np.array([getattr(d, axis) for d in dss]), np.array([getattr(d, name) for d in dss])
+
+
+class EMSimData:
+    """All frequency points of a sweep (emdata.py:246-333)."""
+
+    def __init__(self, basis=None):
+        self._basis = basis
+        self.datasets: list[EMDataSet] = []
+        self._axis = "freq"
+        self.info: dict = {}
+
+    def new(self, **vars) -> EMDataSet:
+        ds = EMDataSet(**vars)
+        self.datasets.append(ds)
+        return ds
+
+    def item(self, k: int) -> EMDataSet:
+        return self.datasets[k]
+
+    def ax(self, field: str) -> _Proxy:
+        return _Proxy(field, self.datasets)
+
+    def __len__(self):
+        return len(self.datasets)
+
+    def S_array(self) -> np.ndarray:
+        """(nfreq, P, P) S matrices."""
+        return np.stack([d.Sp.arry for d in self.datasets])

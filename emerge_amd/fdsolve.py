@@ -1,0 +1,216 @@
+"""Device-resident frequency-domain problem: K(f), port sources, GPU solve, S-parameters.
+
+The reference assembles K(f) = E - k0^2 B + sum_p B_p(f) per frequency on the
+host (fem/physics/edm/assembler.py:312-388), removes PEC DOFs by fancy
+indexing (fem/solver.py:430-435) and solves directly.  Here E and B live on the
+GPU once per mesh; the Robin terms are affine in frequency,
+B_p(f) = gamma_p(f) * P_p and b_p(f) = force_p(f) * q_p, so P_p and q_p are
+built once by the boundary kernel and every frequency costs one affine combine
+plus a scatter of the port blocks.  PEC DOFs are masked inside the solver.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .assembly import GPUPattern
+from .bc import C0, PEC, PortBC, RobinBC
+from .mesh import Mesh3D
+
+# fem/mth/optimized.py:28-29 (order-4 Dunavant rule, 6 points) as gaus_quad_tri(4) lays it out
+_DUNAVANT4 = [(3, 0.223381589678011, 0.108103018168070, 0.445948490915965, 0.445948490915965),
+              (3, 0.109951743655322, 0.816847572980459, 0.091576213509771, 0.091576213509771)]
+
+
+def dunavant4() -> np.ndarray:
+    """(4, 6): weights then barycentric rows (fem/mth/optimized.py:77-108)."""
+    pts = []
+    for N, W, L1, L2, L3 in _DUNAVANT4:
+        l1, l2, l3 = L1, L2, L3
+        for n in range(N):
+            if n == 3:
+                l1, l2, l3 = L1, L3, L2
+            pts.append([W, l1, l2, l3])
+            l1, l2, l3 = l2, l3, l1
+    return np.array(pts).T
+
+
+def pec_dofs(mesh: Mesh3D, bcs) -> np.ndarray:
+    """DOFs on PEC triangles: both modes of their edges and faces (assembler.py:348-359)."""
+    ne, nent = mesh.n_edges, mesh.n_entities
+    out = []
+    for bc in bcs:
+        if isinstance(bc, PEC):
+            tri = mesh.get_triangles(bc.tags)
+            e = np.unique(mesh.tri_to_edge[:, tri].ravel())
+            f = ne + tri
+            ent = np.concatenate([e, f])
+            out.append(np.concatenate([ent, ent + nent]))
+    return np.unique(np.concatenate(out)) if out else np.zeros(0, dtype=np.int64)
+
+
+def tri_dofs(mesh: Mesh3D, tri: np.ndarray) -> np.ndarray:
+    """(ntri, 8) DOFs in the reference's tri slot order (fem/elements/nedelec2.py:57-62)."""
+    ne, nent = mesh.n_edges, mesh.n_entities
+    e = mesh.tri_to_edge[:, tri].T
+    f = (ne + tri)[:, None]
+    return np.concatenate([e, f, e + nent, f + nent], axis=1)
+
+
+def port_local_xy(bc, mesh: Mesh3D, tri: np.ndarray) -> np.ndarray:
+    """(ntri, 6) local vertex coordinates in the frame the reference uses."""
+    v = mesh.nodes[:, mesh.tris[:, tri]]                       # (3, 3 verts, ntri)
+    if bc._include_force:
+        # assembler.py:112: vertices_local = local_basis @ (nodes - origin)
+        B = bc.get_inv_basis()
+        loc = np.einsum("ij,jvt->ivt", B, v - bc.cs.origin[:, None, None])
+        return np.concatenate([loc[0].T, loc[1].T], axis=1)
+    # fem/mth/tri.py:708-721: per-triangle frame (ax1, -axn x ax1, axn) inverted by pinv
+    ax1 = v[:, 1] - v[:, 0]
+    ax2 = v[:, 2] - v[:, 0]
+    ax1 = ax1 / np.linalg.norm(ax1, axis=0)
+    ax2 = ax2 / np.linalg.norm(ax2, axis=0)
+    axn = np.cross(ax1.T, ax2.T).T
+    ax2b = -np.cross(axn.T, ax1.T).T
+    n2 = np.sum(ax2b * ax2b, axis=0)
+    lx = np.einsum("it,ivt->vt", ax1, v)
+    ly = np.einsum("it,ivt->vt", ax2b, v) / n2
+    return np.concatenate([lx.T, ly.T], axis=1)
+
+
+@dataclass
+class RobinTerm:
+    bc: RobinBC
+    tri: np.ndarray
+    pos: object            # device int32 (ntri*64,) CSR positions
+    vals: object           # device c128 (ntri*64,) block values for gamma = 1
+    q: object | None       # device c128 (N,) source for force_scale = 1
+
+
+class FrequencyProblem:
+    """E, B, Robin terms and the PEC mask of one mesh on one GPU."""
+
+    def __init__(self, mesh: Mesh3D, er: np.ndarray, ur: np.ndarray, bcs, exact_face_mass: bool = False,
+                 device: str = "cuda", pattern: GPUPattern | None = None):
+        import torch
+        self.torch = torch
+        self.mesh = mesh
+        self.bcs = list(bcs)
+        self.dev = torch.device(device)
+        self.pat = GPUPattern(mesh, device=device) if pattern is None else pattern
+        self.E, self.B = self.pat.assemble(er, ur, exact_face_mass=exact_face_mass)
+        self.er, self.ur = er, ur
+        self.N, self.nnz = self.pat.n, self.pat.nnz
+        mask = np.ones(self.N, dtype=np.uint8)
+        self.pec = pec_dofs(mesh, self.bcs)
+        mask[self.pec] = 0
+        self.mask = torch.from_numpy(mask).to(self.dev)
+        self.ports = sorted([bc for bc in self.bcs if isinstance(bc, PortBC)], key=lambda b: b.port_number)
+        self.dpts = dunavant4()
+        self._dpts_d = torch.from_numpy(np.ascontiguousarray(self.dpts)).to(self.dev)
+        self.robin = [self._robin_term(bc) for bc in self.bcs if isinstance(bc, RobinBC)]
+        self.K = torch.empty(self.nnz, dtype=torch.complex128, device=self.dev)
+
+    def _robin_term(self, bc) -> RobinTerm:
+        torch, mesh, L = self.torch, self.mesh, _lib.load()
+        tri = mesh.get_triangles(bc.tags)
+        nt = tri.shape[0]
+        lxy = torch.from_numpy(np.ascontiguousarray(port_local_xy(bc, mesh, tri))).to(self.dev)
+        elen = None
+        if not bc._include_force:
+            elen = torch.from_numpy(np.ascontiguousarray(mesh.edge_lengths[mesh.tri_to_edge[:, tri]].T)).to(self.dev)
+        Bblk = torch.empty((nt, 64), dtype=torch.float64, device=self.dev)
+        bblk = None
+        uinc = None
+        if bc._include_force:
+            l = port_local_xy(bc, mesh, tri)
+            xq = l[:, 0:3] @ self.dpts[1:]                        # (nt, nq) local x of the points
+            yq = l[:, 3:6] @ self.dpts[1:]
+            U = bc.mode_shape(xq.ravel(), yq.ravel())               # (3, nt*nq)
+            u = np.stack([U[0].reshape(nt, -1), U[1].reshape(nt, -1)], axis=-1).astype(np.complex128)
+            uinc = torch.from_numpy(np.ascontiguousarray(u)).to(self.dev)
+            bblk = torch.empty((nt, 8), dtype=torch.complex128, device=self.dev)
+        _lib.check(L.emg_robin_blocks(nt, _lib.ptr(lxy), _lib.ptr(elen) if elen is not None else None,
+                                      _lib.ptr(uinc) if uinc is not None else None, self.dpts.shape[1],
+                                      _lib.ptr(self._dpts_d), _lib.ptr(Bblk),
+                                      _lib.ptr(bblk) if bblk is not None else None, _lib.stream_ptr()),
+                   "emg_robin_blocks")
+        dofs = tri_dofs(mesh, tri)                                 # (nt, 8)
+        rows = torch.from_numpy(np.repeat(dofs, 8, axis=1).ravel().astype(np.int32)).to(self.dev)
+        cols = torch.from_numpy(np.tile(dofs, (1, 8)).ravel().astype(np.int32)).to(self.dev)
+        pos = torch.empty(nt * 64, dtype=torch.int32, device=self.dev)
+        _lib.check(L.emg_csr_find(nt * 64, _lib.ptr(self.pat.rowptr), _lib.ptr(self.pat.cols), _lib.ptr(rows),
+                                  _lib.ptr(cols), _lib.ptr(pos), _lib.stream_ptr()), "emg_csr_find")
+        vals = Bblk.reshape(-1).to(torch.complex128)
+        q = None
+        if bblk is not None:
+            q = torch.zeros(self.N, dtype=torch.complex128, device=self.dev)
+            idx = torch.from_numpy(dofs.ravel().astype(np.int32)).to(self.dev)
+            _lib.check(L.emg_scatter_add(nt * 8, _lib.ptr(idx), _lib.ptr(bblk.reshape(-1)), 1.0, 0.0, _lib.ptr(q),
+                                         _lib.stream_ptr()), "emg_scatter_add")
+        return RobinTerm(bc, tri, pos, vals, q)
+
+    def system(self, freq: float, out=None):
+        """K(f) values on the device (complex128, nnz)."""
+        L = _lib.load()
+        k0 = 2 * np.pi * freq / C0
+        K = self.K if out is None else out
+        _lib.check(L.emg_affine_combine(self.nnz, _lib.ptr(self.E), _lib.ptr(self.B), k0 * k0, _lib.ptr(K),
+                                        _lib.stream_ptr()), "emg_affine_combine")
+        for rt in self.robin:
+            if not rt.bc._include_stiff:
+                continue
+            g = complex(rt.bc.get_gamma(k0))
+            _lib.check(L.emg_scatter_add(rt.pos.shape[0], _lib.ptr(rt.pos), _lib.ptr(rt.vals), g.real, g.imag,
+                                         _lib.ptr(K), _lib.stream_ptr()), "emg_scatter_add")
+        return K
+
+    def rhs(self, port: PortBC, freq: float):
+        """Port source vector b_p(f) on the device (masked to free DOFs)."""
+        k0 = 2 * np.pi * freq / C0
+        for rt in self.robin:
+            if rt.bc is port:
+                return rt.q * complex(port.force_scale(k0)) * self.mask
+        raise KeyError(f"port {port.port_number} has no source term")
+
+    def to_scipy(self, vals):
+        return self.pat.to_scipy(vals)
+
+
+class GPUSolver:
+    """Restarted GMRES on the GPU (emg_solver_* in the C ABI)."""
+
+    def __init__(self, problem: FrequencyProblem, restart: int = 150, precision: int = 64):
+        self.L = _lib.load()
+        self.p = problem
+        h = ctypes.c_void_p()
+        _lib.check(self.L.emg_solver_create(problem.N, problem.nnz, _lib.ptr(problem.pat.rowptr),
+                                            _lib.ptr(problem.pat.cols), restart, precision, ctypes.byref(h)),
+                   "emg_solver_create")
+        self.h = h
+        self.precision = precision
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.L.emg_solver_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def set_matrix(self, K):
+        self._K = K
+        _lib.check(self.L.emg_solver_set_matrix(self.h, _lib.ptr(K), _lib.ptr(self.p.mask), _lib.stream_ptr()),
+                   "emg_solver_set_matrix")
+
+    def solve(self, b, x0=None, tol: float = 1e-8, maxiter: int = 20000, max_refine: int = 20):
+        torch = self.p.torch
+        x = torch.zeros_like(b) if x0 is None else x0.clone()
+        info = _lib.SolveInfo()
+        _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, maxiter, max_refine,
+                                           ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
+        return x, info

@@ -1,0 +1,170 @@
+"""Electrodynamics3D: the frequency-domain / eigenmode driver (fem/physics/edm/emfreq3d.py).
+
+Same call pattern as the reference -- set_frequency(), assign(*bcs),
+frequency_domain() -> EMSimData, plus the 3D eigenmode() the reference has
+commented out (emfreq3d.py:815-858) -- on an emerge_amd Mesh3D with materials
+per volume tag instead of a gmsh session.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .bc import AIR, C0, PEC, LumpedPort, Material, PortBC
+from .emdata import EMSimData, SolveRecord
+from .mesh import Mesh3D
+
+
+class SimulationError(Exception):
+    """emfreq3d.py:37-38."""
+
+
+class GPUSolveSettings:
+    """Solver knobs (the reference's SolveRoutine flags, fem/solver.py:361-402, for the GPU path).
+
+    precision: 64 (complex128 GMRES) or 32 (complex64 GMRES in FP64 refinement)
+    tol: FP64 relative residual target ||b - K x|| / ||b|| over free DOFs
+    """
+
+    def __init__(self, precision: int = 32, tol: float = 1e-8, restart: int = 150, maxiter: int = 60000,
+                 max_refine: int = 30, warm_start: bool = True, store_fields: bool = True):
+        self.precision = precision
+        self.tol = tol
+        self.restart = restart
+        self.maxiter = maxiter
+        self.max_refine = max_refine
+        self.warm_start = warm_start
+        self.store_fields = store_fields
+
+
+class Electrodynamics3D:
+    """Time-harmonic Nedelec-2 physics on one GPU (emfreq3d.py:74-807)."""
+
+    def __init__(self, mesh: Mesh3D, order: int = 2, materials: dict[int, Material] | None = None):
+        self.frequencies: list[float] = []
+        self.mesh = mesh
+        self.set_order(order)
+        self.materials: dict[int, Material] = dict(materials or {})
+        self.solveroutine = GPUSolveSettings()
+        self.exact_face_mass = False
+        self.data: EMSimData | None = None
+        self._problem = None
+        self._problem_key = None
+        self._bc_initialized = False
+        self.boundary_conditions = []
+        self._initialize_bcs()
+
+    def set_order(self, order: int) -> None:                                      # emfreq3d.py:100-114
+        if order not in (2,):
+            raise ValueError(f"Order {order} not supported. Only order-2 allowed.")
+        self.order = order
+        self.resolution = 0.3
+
+    def _initialize_bcs(self) -> None:                                           # emfreq3d.py:137-147
+        self.boundary_conditions = [PEC(sorted(self.mesh.face_tags.keys()))]
+
+    def set_frequency(self, frequency) -> None:                                  # emfreq3d.py:149-164
+        if isinstance(frequency, (tuple, list, np.ndarray)):
+            self.frequencies = [float(f) for f in frequency]
+        else:
+            self.frequencies = [float(frequency)]
+
+    def set_material(self, vol_tags, material: Material) -> None:
+        for t in ([vol_tags] if np.isscalar(vol_tags) else vol_tags):
+            self.materials[int(t)] = material
+
+    @property
+    def nports(self) -> int:
+        return len([bc for bc in self.boundary_conditions if isinstance(bc, PortBC)])
+
+    def ports(self) -> list[PortBC]:
+        return sorted([bc for bc in self.boundary_conditions if isinstance(bc, PortBC)], key=lambda b: b.port_number)
+
+    def assign(self, *bcs) -> None:                                              # emfreq3d.py:782-807
+        self._bc_initialized = True
+        for bc in bcs:
+            for existing in self.boundary_conditions:
+                existing.exclude_bc(bc)
+            self.boundary_conditions.append(bc)
+        self._problem = None
+
+    def material_tensors(self):
+        """(3,3,nt) complex eps_r and mu_r per tet (fem/mesh3d.py:358-378)."""
+        nt = self.mesh.n_tets
+        er = np.zeros((3, 3, nt), dtype=np.complex128)
+        ur = np.zeros((3, 3, nt), dtype=np.complex128)
+        tags = self.mesh.tet_tags
+        for tag in np.unique(tags):
+            mat = self.materials.get(int(tag), AIR)
+            ids = tags == tag
+            er[:, :, ids] = mat.ermat[:, :, None]
+            ur[:, :, ids] = mat.urmat[:, :, None]
+        return er, ur
+
+    def problem(self):
+        """The device-resident frequency problem (built once per mesh/material/BC set)."""
+        from .fdsolve import FrequencyProblem
+        key = (id(self.mesh), tuple(sorted((k, v.er, v.ur, v.tand) for k, v in self.materials.items())),
+               tuple(id(b) for b in self.boundary_conditions), self.exact_face_mass)
+        if self._problem is None or self._problem_key != key:
+            er, ur = self.material_tensors()
+            for p in self.ports():
+                if isinstance(p, LumpedPort) and p.vintline is None:
+                    from .sparam import lumped_port_line
+                    lumped_port_line(self.mesh, p)
+            self._problem = FrequencyProblem(self.mesh, er, ur, self.boundary_conditions,
+                                             exact_face_mass=self.exact_face_mass)
+            self._problem_key = key
+        return self._problem
+
+    def frequency_domain(self) -> EMSimData:
+        """Sweep all frequencies and ports on the GPU (emfreq3d.py:607-732)."""
+        if self._bc_initialized is False:
+            raise SimulationError("Cannot run a modal analysis because no boundary conditions have been assigned.")
+        if not self.frequencies:
+            raise SimulationError("No frequencies set (set_frequency).")
+        import torch
+        from .fdsolve import GPUSolver
+        from .sparam import PortSampler
+        st = self.solveroutine
+        prob = self.problem()
+        sampler = PortSampler(prob)
+        solver = GPUSolver(prob, restart=st.restart, precision=st.precision)
+        ports = prob.ports
+        nums = [p.port_number for p in ports]
+        data = EMSimData()
+        er, ur = prob.er, prob.ur
+        prev: dict[int, object] = {}
+        t0 = time.perf_counter()
+        for freq in self.frequencies:
+            k0 = 2 * np.pi * freq / C0
+            ds = data.new(freq=freq, k0=k0)
+            ds.init_sp(nums)
+            ds.er = np.squeeze(er[0, 0, :])
+            ds.ur = np.squeeze(ur[0, 0, :])
+            for p in ports:
+                ds.add_port_properties(p.port_number, p.mode_number, k0, p.get_beta(k0), p.Z0, p.power)
+            K = prob.system(freq)
+            solver.set_matrix(K)
+            X = []
+            for ap in ports:
+                b = prob.rhs(ap, freq)
+                x0 = prev.get(ap.port_number) if st.warm_start else None
+                x, info = solver.solve(b, x0=x0, tol=st.tol, maxiter=st.maxiter, max_refine=st.max_refine)
+                ds.solves.append(SolveRecord(ap.port_number, info.iterations, info.refinements, info.residual,
+                                             bool(info.converged), f"gmres-c{64 if st.precision == 32 else 128}"))
+                prev[ap.port_number] = x
+                X.append(x)
+            Xs = torch.stack(X)
+            E = sampler.fields(Xs)
+            S = sampler.s_matrix({p.port_number: E[i] for i, p in enumerate(ports)}, k0)
+            ds.Sp.arry[:, :] = S
+            if st.store_fields:
+                for i, p in enumerate(ports):
+                    ds._fields[p.port_number] = X[i].cpu().numpy()
+            ds.set_field_vector()
+        torch.cuda.synchronize()
+        data.info["sweep_seconds"] = time.perf_counter() - t0
+        self.data = data
+        return data

@@ -75,6 +75,65 @@ int emg_assemble_tensor(const emg_pattern *pat, const double *d_nodes, const int
                         const double *d_mm, const double *d_ms, int flags,
                         double *d_E, double *d_B, void *stream);
 
+/* ------------------------------------------------------------------ solver
+ * Right-preconditioned restarted GMRES for K x = b on a pattern's CSR,
+ * replacing SolveRoutine.solve (fem/solver.py:405-469).  PEC DOFs (mask 0)
+ * are eliminated implicitly (x = 0 there, as solver.py:429,465 leaves them).
+ * precision 64: complex128 GMRES.  precision 32: complex64 GMRES inside FP64
+ * iterative refinement; the returned residual is always the FP64
+ * ||b - K x|| / ||b|| over the free DOFs.  Preconditioner: inverse 2x2 blocks
+ * coupling the two modes of every edge / face (entity e <-> e + N/2).       */
+typedef struct emg_solver emg_solver;
+typedef struct {
+    int iterations;   /* Krylov iterations over all restarts / refinements */
+    int refinements;  /* FP64 refinement steps (precision 32) */
+    int converged;    /* residual <= tol */
+    int reserved;
+    double residual;  /* FP64 relative residual */
+} emg_solve_info;
+
+int emg_solver_create(int64_t n, int64_t nnz, const int32_t *d_rowptr, const int32_t *d_cols, int restart,
+                      int precision, emg_solver **out);
+/* K values (nnz complex128, caller-owned, must outlive the solves) and the
+ * free-DOF mask (n uint8, 1 = solve, 0 = PEC; NULL = all free). */
+int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const uint8_t *d_mask, void *stream);
+/* d_x: initial guess in, solution out (n complex128). Synchronises `stream`. */
+int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, double tol, int maxiter, int max_refine,
+                     emg_solve_info *info, void *stream);
+void emg_solver_destroy(emg_solver *s);
+
+/* K = E - s*B on the pattern (complex128 values, nnz each). */
+int emg_affine_combine(int64_t nnz, const double *d_E, const double *d_B, double s, double *d_K, void *stream);
+/* d_pos[i] = CSR position of (d_qr[i], d_qc[i]) or -1 (binary search in sorted rows). */
+int emg_csr_find(int64_t m, const int32_t *d_rowptr, const int32_t *d_cols, const int32_t *d_qr, const int32_t *d_qc,
+                 int32_t *d_pos, void *stream);
+/* d_K[d_pos[i]] += (gre + j gim) * d_v[i] for i < m (complex128, atomic). */
+int emg_scatter_add(int64_t m, const int32_t *d_pos, const double *d_v, double gre, double gim, double *d_K, void *stream);
+
+/* ------------------------------------------------------------------ boundary terms
+ * Robin (port / absorbing boundary) triangle terms, replacing
+ * ned2_tri_stiff_force and ned2_tri_stiff (fem/mth/tri.py:575-796) as called
+ * from assemble_robin_bc_excited / assemble_robin_bc
+ * (fem/physics/edm/assembler.py:100-144).
+ *   d_lxy  (ntri, 6) f64: local x0,x1,x2,y0,y1,y2 of each triangle (vertices in
+ *          ascending global id), in the frame the reference uses;
+ *   d_elen (ntri, 3) f64 edge lengths of edges (0,1),(1,2),(0,2), or NULL to use
+ *          the local distances (force variant);
+ *   d_uinc (ntri, nq, 2) c128 incident field (x, y local) at the nq points of
+ *          d_dpts (4, nq: weights, then barycentric rows), or NULL;
+ *   d_B    (ntri, 64) f64: the 8x8 block for gamma = 1;
+ *   d_b    (ntri, 8) c128 source vector, or NULL.                             */
+int emg_robin_blocks(int64_t ntri, const double *d_lxy, const double *d_elen, const double *d_uinc, int nq,
+                     const double *d_dpts, double *d_B, double *d_b, void *stream);
+
+/* Nedelec-2 field of nsol solution vectors (d_x: nsol x N complex128) at npts
+ * points given by tet id (d_ptet) and barycentric coordinates (d_bary, npts x 4,
+ * canonical vertex order).  d_E: nsol x npts x 3 complex128.  Replaces
+ * ned2_tet_interp (fem/mth/tet.py:371-497).                                   */
+int emg_interp_points(int64_t npts, const int32_t *d_ptet, const double *d_bary, const double *d_nodes,
+                      const int32_t *d_tets, const int32_t *d_tet_ent, int64_t n_ent, int nsol,
+                      const double *d_x, double *d_E, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

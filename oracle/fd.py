@@ -1,0 +1,262 @@
+"""CPU ORACLE -- test infrastructure only.
+
+Restatement of the reference's frequency-domain path on the CPU:
+Robin / port terms and PEC elimination (fem/physics/edm/assembler.py:63-144,
+312-388), the FP64 direct solve (fem/solver.py:405-469; scipy SuperLU in
+place of PARDISO, identical in exact arithmetic), field interpolation
+(fem/mth/tet.py:371-497) and S-parameter extraction
+(fem/physics/edm/emfreq3d.py:607-779, fem/mth/sparam.py:72-139,
+fem/mth/integrals.py:24-70).  Boundary-condition objects come from
+``emerge_amd.bc`` (pure data + the reference's port formulas).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from . import oracle as O
+
+C0 = 299792458.0
+
+# fem/mth/optimized.py:28-29 (order-4 Dunavant rule) and :77-108 gaus_quad_tri
+_DUNAVANT4 = [(3, 0.223381589678011, 0.108103018168070, 0.445948490915965, 0.445948490915965),
+              (3, 0.109951743655322, 0.816847572980459, 0.091576213509771, 0.091576213509771)]
+
+
+def gaus_quad_tri4() -> np.ndarray:
+    pts = []
+    for N, W, L1, L2, L3 in _DUNAVANT4:
+        l1, l2, l3 = L1, L2, L3
+        for n in range(N):
+            if n == 3:
+                l1, l2, l3 = L1, L3, L2
+            pts.append([W, l1, l2, l3])
+            l1, l2, l3 = l2, l3, l1
+    return np.array(pts).T
+
+
+def _tri_stiff_force(lcs, gamma, Uinc, dpts):
+    B = np.empty((8, 8), dtype=np.complex128)
+    b = np.empty(8, dtype=np.complex128)
+    O.lib().oracle_ned2_tri_stiff_force(O._p(np.ascontiguousarray(lcs, dtype=np.float64)), float(np.real(gamma)),
+                                        float(np.imag(gamma)), O._p(np.ascontiguousarray(Uinc, dtype=np.complex128)),
+                                        O._p(np.ascontiguousarray(dpts)), dpts.shape[1], O._p(B), O._p(b))
+    return B, b
+
+
+def _tri_stiff(verts, edge_lengths, gamma):
+    B = np.empty((8, 8), dtype=np.complex128)
+    O.lib().oracle_ned2_tri_stiff(O._p(np.ascontiguousarray(verts, dtype=np.float64)),
+                                  O._p(np.ascontiguousarray(edge_lengths, dtype=np.float64)),
+                                  float(np.real(gamma)), float(np.imag(gamma)), O._p(B))
+    return B
+
+
+def _tri_csr(field, tri_ids, blocks, N):
+    rows, cols, vals = [], [], []
+    for i, itri in enumerate(tri_ids):
+        idx = field.tri_to_field[:, itri]
+        rows.append(np.repeat(idx, 8))
+        cols.append(np.tile(idx, 8))
+        vals.append(blocks[i].ravel())
+    if not rows:
+        return sp.csr_matrix((N, N), dtype=np.complex128)
+    return sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N)).tocsr()
+
+
+class OracleProblem:
+    """Frequency-domain problem on the oracle's own topology/numbering.
+
+    mesh: an emerge_amd Mesh3D (for face tags); the oracle rebuilds topology
+    with OracleMesh(order_seed) and maps tagged triangles into it.
+    """
+
+    def __init__(self, mesh, er, ur, bcs, order_seed=None, E=None, B=None):
+        self.src = mesh
+        self.om = O.OracleMesh(mesh.nodes, mesh.tets, order_seed=order_seed)
+        self.field = O.OracleNedelec2(self.om)
+        self.er, self.ur = er, ur
+        self.bcs = bcs
+        if E is None:
+            E, B = O.tet_mass_stiffness_matrices(self.field, er, ur)
+        self.E, self.B = E, B
+        self.N = self.field.n_field
+
+    def tri_ids(self, tags):
+        src = self.src
+        ids = src.get_triangles(tags)
+        return self.om.get_tri_ids(src.tris[:, ids])
+
+    # fem/physics/edm/assembler.py:312-388
+    def assemble_freq_matrix(self, frequency):
+        k0 = 2 * np.pi * frequency / C0
+        K = (self.E - self.B * (k0 ** 2)).tocsr()
+        from emerge_amd.bc import PEC, RobinBC, PortBC
+        pec_ids = []
+        for bc in self.bcs:
+            if isinstance(bc, PEC):
+                tri = self.tri_ids(bc.tags)
+                for e in self.om.tri_to_edge[:, tri].ravel():
+                    pec_ids.extend(self.field.edge_to_field[:, e])
+                for t in tri:
+                    pec_ids.extend(self.field.tri_to_field[:, t])
+        b = np.zeros(self.N, dtype=np.complex128)
+        port_vectors = {bc.port_number: np.zeros(self.N, dtype=np.complex128) for bc in self.bcs if isinstance(bc, PortBC)}
+        dpts = gaus_quad_tri4()
+        for bc in self.bcs:
+            if not isinstance(bc, RobinBC):
+                continue
+            tri = self.tri_ids(bc.tags)
+            gamma = bc.get_gamma(k0)
+            if bc._include_force:
+                # assembler.py:100-123
+                nodes = self.om.nodes
+                vl = bc.get_inv_basis() @ (nodes - bc.cs.origin[:, None])
+                blocks = []
+                bvec = np.zeros(self.N, dtype=np.complex128)
+                for itri in tri:
+                    vid = self.om.tris[:, itri]
+                    x = vl[0, vid] @ dpts[1:]
+                    y = vl[1, vid] @ dpts[1:]
+                    U = bc.get_Uinc(x, y, k0)
+                    Bs, bs = _tri_stiff_force(vl[:, vid], gamma, U, dpts)
+                    blocks.append(Bs)
+                    np.add.at(bvec, self.field.tri_to_field[:, itri], bs)
+                port_vectors[bc.port_number] += bvec
+            else:
+                # assembler.py:125-144
+                blocks = [_tri_stiff(self.om.nodes[:, self.om.tris[:, t]], self.om.edge_lengths[self.om.tri_to_edge[:, t]], gamma)
+                          for t in tri]
+            if bc._include_stiff:
+                K = K + _tri_csr(self.field, tri, blocks, self.N)
+        pec = np.unique(np.asarray(pec_ids, dtype=np.int64))
+        solve_ids = np.setdiff1d(np.arange(self.N), pec)
+        return K.tocsr(), b, solve_ids, port_vectors
+
+    # fem/solver.py:405-469 (direct solve on the PEC-reduced system) with the
+    # reference's SuperLU settings (solver.py:243-275: MMD_AT_PLUS_A ordering,
+    # diag_pivot_thresh 0.01, SymmetricMode); the factorisation of the last
+    # matrix is reused for the second port like reuse_factorization.
+    def solve(self, K, b, solve_ids):
+        x = np.zeros(K.shape[0], dtype=np.complex128)
+        key = id(K)
+        if getattr(self, "_lu_key", None) != key:
+            A = K[solve_ids][:, solve_ids].tocsc()
+            self._lu = spla.splu(A, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.01,
+                                 options=dict(SymmetricMode=True))
+            self._lu_key = key
+        x[solve_ids] = self._lu.solve(b[solve_ids])
+        return x
+
+    # fem/mth/tet.py:371-497 evaluated in the tets that hold the points
+    def interpolate(self, x, pts, tet_ids=None):
+        om, f = self.om, self.field
+        nodes = om.nodes
+        ne = om.n_edges
+        out = np.zeros((3, pts.shape[1]), dtype=np.complex128)
+        if tet_ids is None:
+            tet_ids = np.arange(om.n_tets)
+        for itet in tet_ids:
+            vid = om.tets[:, itet]
+            v = nodes[:, vid]
+            T = np.stack([v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]], axis=1)
+            loc = np.linalg.pinv(T) @ (pts - v[:, :1])
+            inside = (loc.sum(0) <= 1.00000001) & np.all(loc >= -1e-6, axis=0)
+            if not inside.any():
+                continue
+            P = pts[:, inside]
+            lam = np.vstack([1 - loc[:, inside].sum(0), loc[:, inside]])
+            G = np.linalg.inv(T)                 # rows: grad of local coords
+            G = np.vstack([-G.sum(0), G])        # (4,3) grad lambda
+            Et = x[f.tet_to_field[:, itet]]
+            lem = O.local_mapping(vid, om.edges[:, f.tet_to_field[:6, itet]])
+            ltm = O.local_mapping(vid, om.tris[:, f.tet_to_field[6:10, itet] - ne])
+            E = np.zeros((3, P.shape[1]), dtype=np.complex128)
+            for ie in range(6):
+                a, b_ = lem[:, ie]
+                L = np.linalg.norm(v[:, a] - v[:, b_])
+                w = lam[b_] * G[a][:, None] - lam[a] * G[b_][:, None]
+                E += L * (Et[ie] * lam[a] + Et[10 + ie] * lam[b_]) * w
+            for ie in range(4):
+                a, b_, c = ltm[:, ie]
+                L1 = np.linalg.norm(v[:, a] - v[:, c])
+                L2 = np.linalg.norm(v[:, a] - v[:, b_])
+                E += -Et[6 + ie] * L1 * (G[a][:, None] * lam[c] - G[c][:, None] * lam[a]) * lam[b_]
+                E += Et[16 + ie] * L2 * (G[a][:, None] * lam[b_] - G[b_][:, None] * lam[a]) * lam[c]
+            out[:, inside] = E
+        return out
+
+    # fem/physics/edm/emfreq3d.py:734-779
+    def s_data(self, bc, x, k0):
+        if bc.v_integration:
+            ln = bc.vintline
+            pts = np.array(ln.cmid)
+            Ex, Ey, Ez = self.interpolate(x, pts, self._candidate_tets(pts))
+            V = np.sum(Ex * ln.dxs + Ey * ln.dys + Ez * ln.dzs)
+            if bc.active:
+                a, b = bc.voltage, V - bc.voltage
+            else:
+                a, b = 0, V
+            return np.sqrt(b ** 2 / (2 * bc.Z0)), np.sqrt(a ** 2 / (2 * bc.Z0))
+        tri = self.tri_ids(bc.tags)
+        tv = self.om.tris[:, tri]
+        src_tri = self.src.get_triangles(bc.tags)
+        tets_src = self.src.tri_to_tet[0, src_tri]
+        er = self.er[:, :, tets_src]
+        ur = self.ur[:, :, tets_src]
+        const = 1 / np.sqrt((ur[0, 0] + ur[1, 1] + ur[2, 2]) / (er[0, 0] + er[1, 1] + er[2, 2]))   # TEM
+        dpts = gaus_quad_tri4()
+        nodes = self.om.nodes
+        P = np.einsum("cvt,vq->cqt", nodes[:, tv], dpts[1:])          # (3, nq, ntri)
+        pts = P.reshape(3, -1)
+        Ef = self.interpolate(x, pts, self._candidate_tets(pts)).reshape(3, dpts.shape[1], -1)
+        Em = bc.port_mode_3d_global(pts[0], pts[1], pts[2], k0).reshape(3, dpts.shape[1], -1)
+        Q = 1 if bc.active else 0
+        Zm = bc.Zmode(k0)
+        A = self.om.areas[tri] if hasattr(self.om, "areas") else None
+        fp = np.sum((Ef - Q * Em) * np.conj(Em), axis=0) / (2 * Zm)    # (nq, ntri)
+        mp = np.sum(Em * np.conj(Em), axis=0) / (2 * Zm)
+        w = dpts[0]
+        field_p = np.sum(const * A * (w @ fp))
+        mode_p = np.sum(const * A * (w @ mp))
+        return field_p, mode_p
+
+    def _candidate_tets(self, pts):
+        """Tets touching any vertex of the port faces (the reference's all_port_tets)."""
+        if not hasattr(self, "_port_tets"):
+            from emerge_amd.bc import PortBC
+            verts = set()
+            for bc in self.bcs:
+                if isinstance(bc, PortBC):
+                    verts.update(self.om.tris[:, self.tri_ids(bc.tags)].ravel().tolist())
+            mask = np.isin(self.om.tets, np.array(sorted(verts))).any(axis=0)
+            self._port_tets = np.flatnonzero(mask)
+        return self._port_tets
+
+    # fem/physics/edm/emfreq3d.py:607-732
+    def frequency_domain(self, frequencies):
+        from emerge_amd.bc import PortBC
+        ports = [bc for bc in self.bcs if isinstance(bc, PortBC)]
+        nums = [p.port_number for p in ports]
+        S = np.zeros((len(frequencies), len(ports), len(ports)), dtype=np.complex128)
+        sols = []
+        for fi, freq in enumerate(frequencies):
+            k0 = 2 * np.pi * freq / C0
+            K, b, solve_ids, pv = self.assemble_freq_matrix(freq)
+            fsol = {}
+            for p in ports:
+                p.active = False
+            for ap in ports:
+                ap.active = True
+                x = self.solve(K, b + pv[ap.port_number], solve_ids)
+                fsol[ap.port_number] = x
+                _, pmode = self.s_data(ap, x, k0)
+                for bc in ports:
+                    pfield, _ = self.s_data(bc, x, k0)
+                    S[fi, nums.index(bc.port_number), nums.index(ap.port_number)] = pfield / pmode
+                ap.active = False
+            sols.append(fsol)
+        return S, sols

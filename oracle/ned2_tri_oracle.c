@@ -78,8 +78,9 @@ static void tri_block(const double *xs, const double *ys, const double *Le, doub
  * lcs (3,3) row-major local vertex coords (only x and y rows used);
  * Uinc (3, nq) row-major local incident field at the nq quadrature points;
  * dpts (4, nq) row-major: weights, then the three barycentric rows.          */
-void oracle_ned2_tri_stiff_force(const double *lcs, cplx gamma, const cplx *Uinc, const double *dpts, int nq,
-                                 cplx *Bmat, cplx *bvec) {
+void oracle_ned2_tri_stiff_force(const double *lcs, double gre, double gim, const cplx *Uinc, const double *dpts,
+                                 int nq, cplx *Bmat, cplx *bvec) {
+    const cplx gamma = gre + I * gim;
     const int lem[2][3] = {{0, 1, 0}, {1, 2, 2}};
     const double *xs = lcs, *ys = lcs + 3;
     double x1 = xs[0], x2 = xs[1], x3 = xs[2], y1 = ys[0], y2 = ys[1], y3 = ys[2];
@@ -127,7 +128,8 @@ void oracle_ned2_tri_stiff_force(const double *lcs, cplx gamma, const cplx *Uinc
 
 /* fem/mth/tri.py:697-796  ned2_tri_stiff: projects the 3D triangle into its
  * own plane first (:708-721), edge lengths are the mesh's (:740).           */
-void oracle_ned2_tri_stiff(const double *verts, const double *edge_lengths, cplx gamma, cplx *Bmat) {
+void oracle_ned2_tri_stiff(const double *verts, const double *edge_lengths, double gre, double gim, cplx *Bmat) {
+    const cplx gamma = gre + I * gim;
     const double *xs = verts, *ys = verts + 3, *zs = verts + 6;
     double ax1[3] = {xs[1] - xs[0], ys[1] - ys[0], zs[1] - zs[0]};
     double ax2[3] = {xs[2] - xs[0], ys[2] - ys[0], zs[2] - zs[0]};

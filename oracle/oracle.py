@@ -50,9 +50,10 @@ def lib() -> ctypes.CDLL:
             fn.restype = None
         L.oracle_matinv.argtypes = [P, P]
         L.oracle_matinv.restype = None
-        L.oracle_ned2_tri_stiff_force.argtypes = [P, ctypes.c_double * 2, P, P, ctypes.c_int, P, P]
+        D = ctypes.c_double
+        L.oracle_ned2_tri_stiff_force.argtypes = [P, D, D, P, P, ctypes.c_int, P, P]
         L.oracle_ned2_tri_stiff_force.restype = None
-        L.oracle_ned2_tri_stiff.argtypes = [P, P, ctypes.c_double * 2, P]
+        L.oracle_ned2_tri_stiff.argtypes = [P, P, D, D, P]
         L.oracle_ned2_tri_stiff.restype = None
         _LIB = L
     return _LIB
