@@ -33,6 +33,10 @@ SIGNATURES: dict[str, tuple] = {
     "emg_affine_combine": (ctypes.c_int, [I64, P, P, F64, P, P]),
     "emg_csr_find": (ctypes.c_int, [I64, P, P, P, P, P, P]),
     "emg_scatter_add": (ctypes.c_int, [I64, P, P, F64, F64, P, P]),
+    "emg_spmv": (ctypes.c_int, [I64, P, P, P, P, P, P, P]),
+    "emg_mdotc": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P]),
+    "emg_maxpy": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P, P]),
+    "emg_mcomb": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P]),
     "emg_robin_blocks": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P, P]),
     "emg_interp_points": (ctypes.c_int, [I64, P, P, P, P, P, I64, ctypes.c_int, P, P, P]),
 }

@@ -603,28 +603,35 @@ extern "C" int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, d
         if (rc) return rc;
         info->residual = rel;
     } else {
-        // FP64 iterative refinement around complex64 GMRES: each correction
-        // solves K d = r / |r| in complex64, the residual is recomputed in FP64.
+        // Mixed-precision restarted GMRES: every restart cycle runs in complex64
+        // on K d = r / |r|, then x += d and r = b - K x are formed in FP64, so
+        // complex64 never has to resolve more than one cycle's reduction.
         float *r32 = reinterpret_cast<float *>(s->d);
         float *d32 = reinterpret_cast<float *>(s->d) + 2 * n;
         int rc = true_resid(&rel, &bn);
         if (rc) return rc;
         int total = 0, ref = 0;
-        while (rel > tol && ref < max_refine && total < maxiter) {
+        const int limit = max_refine > 0 ? max_refine : (1 << 30);
+        int stall = 0;
+        while (rel > tol && ref < limit && total < maxiter) {
             const double rs = 1.0 / (rel * bn);
             k_c128_to_c64_scaled<<<1024, RB, 0, st>>>(n, s->r, rs, r32);
             EMG_HIP(hipMemsetAsync(d32, 0, n * 8, st));
-            const double inner_tol = std::max(0.5 * tol / rel, 2e-6);
+            const double inner_tol = std::max(0.5 * tol / rel, 1e-7);
             int it;
             double est;
-            rc = gmres_core<float>(s, s->vals32, s->inv32, r32, d32, inner_tol, maxiter - total, &it, &est, st);
+            rc = gmres_core<float>(s, s->vals32, s->inv32, r32, d32, inner_tol,
+                                   std::min(s->restart, maxiter - total), &it, &est, st);
             if (rc) return rc;
             total += it;
             ref++;
             k_add_scaled<float><<<1024, RB, 0, st>>>(n, d32, 1.0 / rs, d_x);
+            const double prev = rel;
             rc = true_resid(&rel, &bn);
             if (rc) return rc;
             if (it == 0) break;
+            stall = (rel > 0.999 * prev) ? stall + 1 : 0;
+            if (stall >= 3) break;   // no progress in complex64: stop (caller sees converged = 0)
         }
         info->iterations = total;
         info->refinements = ref;
@@ -656,6 +663,52 @@ extern "C" int emg_scatter_add(int64_t m, const int32_t *d_pos, const double *d_
     EMG_CHECK(d_pos && d_v && d_K, "emg_scatter_add: null argument");
     if (m == 0) return 0;
     k_scatter_add<<<emg::grid_for(m, RB), RB, 0, (hipStream_t)stream>>>(m, d_pos, d_v, gre, gim, d_K);
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+// ------------------------------------------------------------------ vector primitives
+// complex128 building blocks for the eigen driver (Arnoldi with CGS2).
+extern "C" int emg_spmv(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_vals,
+                        const uint8_t *d_mask, const double *d_x, double *d_y, void *stream) {
+    EMG_CHECK(d_rowptr && d_cols && d_vals && d_x && d_y, "emg_spmv: null argument");
+    k_spmv<double><<<(unsigned)((n * SUB + 255) / 256), 256, 0, (hipStream_t)stream>>>(n, d_rowptr, d_cols, d_vals,
+                                                                                     d_mask, d_x, nullptr, d_y);
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+static int nblk_for(int64_t n) { return (int)std::min<int64_t>((n + RB - 1) / RB, 1024); }
+
+// d_h[i] = sum_k conj(V_i[k]) w[k], i < nv; V holds nv vectors of n entries back to back.
+// d_work: at least ceil(nv/8) * 1024 * 16 + 16 doubles.
+extern "C" int emg_mdotc(int64_t n, int nv, const double *d_V, const double *d_w, double *d_h, double *d_work,
+                         void *stream) {
+    EMG_CHECK(d_V && d_w && d_h && d_work && nv > 0 && nv <= 512, "emg_mdotc: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = nblk_for(n);
+    k_mdot<double><<<dim3(nb, (nv + DT - 1) / DT), RB, 0, st>>>(n, nv, d_V, n, d_w, d_work);
+    k_mdot_finish<<<nv, 256, 0, st>>>(nv, nb, d_work, d_h, 0);
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+// w -= sum_i h_i V_i ; *d_norm (on device) = ||w|| after the update.
+extern "C" int emg_maxpy(int64_t n, int nv, const double *d_V, const double *d_h, double *d_w, double *d_norm,
+                         double *d_work, void *stream) {
+    EMG_CHECK(d_V && d_h && d_w && d_norm && d_work && nv >= 0 && nv <= 512, "emg_maxpy: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = nblk_for(n);
+    k_maxpy_norm<double><<<nb, RB, 0, st>>>(n, nv, d_V, n, d_h, d_w, d_work);
+    k_scale_by_norm<double><<<1, RB, 0, st>>>(0, d_work, nb, d_w, d_w, d_norm);   // n = 0: norm only
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+// x = sum_i y_i V_i  (y complex on device)
+extern "C" int emg_mcomb(int64_t n, int nv, const double *d_V, const double *d_y, double *d_x, void *stream) {
+    EMG_CHECK(d_V && d_y && d_x && nv > 0 && nv <= 512, "emg_mcomb: bad argument");
+    k_mcomb<double><<<nblk_for(n), RB, 0, (hipStream_t)stream>>>(n, nv, d_V, n, d_y, d_x, 1);
     EMG_LAUNCH_CHECK();
     return 0;
 }

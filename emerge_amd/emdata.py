@@ -73,6 +73,7 @@ class EMDataSet:
         self.solves: list[SolveRecord] = []
         self.mode = None
         self.beta = None
+        self._mode_field = None
         for k, v in kw.items():
             setattr(self, k, v)
 
@@ -95,6 +96,8 @@ class EMDataSet:
 
     @property
     def _field(self):
+        if self._mode_field is not None:
+            return self._mode_field
         return sum(self.excitation[m.port_number] * self._fields[m.port_number] for m in self.port_modes
                    if m.port_number in self._fields)
 

@@ -207,7 +207,7 @@ class GPUSolver:
         _lib.check(self.L.emg_solver_set_matrix(self.h, _lib.ptr(K), _lib.ptr(self.p.mask), _lib.stream_ptr()),
                    "emg_solver_set_matrix")
 
-    def solve(self, b, x0=None, tol: float = 1e-8, maxiter: int = 20000, max_refine: int = 20):
+    def solve(self, b, x0=None, tol: float = 1e-8, maxiter: int = 20000, max_refine: int = 0):
         torch = self.p.torch
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         info = _lib.SolveInfo()

@@ -25,10 +25,11 @@ class GPUSolveSettings:
 
     precision: 64 (complex128 GMRES) or 32 (complex64 GMRES in FP64 refinement)
     tol: FP64 relative residual target ||b - K x|| / ||b|| over free DOFs
+    max_refine: cap on FP64 refinement cycles for precision 32 (0 = no cap besides maxiter)
     """
 
     def __init__(self, precision: int = 32, tol: float = 1e-8, restart: int = 150, maxiter: int = 60000,
-                 max_refine: int = 30, warm_start: bool = True, store_fields: bool = True):
+                 max_refine: int = 0, warm_start: bool = True, store_fields: bool = True):
         self.precision = precision
         self.tol = tol
         self.restart = restart
@@ -166,5 +167,31 @@ class Electrodynamics3D:
             ds.set_field_vector()
         torch.cuda.synchronize()
         data.info["sweep_seconds"] = time.perf_counter() - t0
+        self.data = data
+        return data
+
+    def eigenmode(self, num_sols: int = 6, target_frequency: float | None = None, ncv: int | None = None,
+                  inner_tol: float = 1e-12, spurious_rel: float = 1e-3) -> EMSimData:
+        """Modes of E x = k^2 B x nearest k(target)^2 with PEC walls (emfreq3d.py:815-858).
+
+        Returns an EMSimData with one dataset per mode: freq (Hz), k0, mode,
+        residual, and the mode field as _mode_field."""
+        from .eigen import ShiftInvertArnoldi, eigen_frequencies
+        f_t = target_frequency if target_frequency is not None else (self.frequencies[0] if self.frequencies else None)
+        if f_t is None:
+            raise SimulationError("eigenmode needs a target frequency")
+        sigma = (2 * np.pi * f_t / C0) ** 2                                      # emfreq3d.py:838-839
+        prob = self.problem()
+        arn = ShiftInvertArnoldi(prob, sigma, ncv=ncv or max(2 * num_sols + 10, 24), inner_tol=inner_tol)
+        lam, X, resid = arn.run(num_sols, spurious_rel=spurious_rel)
+        data = EMSimData()
+        for i in range(lam.shape[0]):
+            f = float(eigen_frequencies(lam[i].real))
+            ds = data.new(freq=f, k0=float(np.sqrt(lam[i].real)), mode=i + 1)
+            ds.eigenvalue = complex(lam[i])
+            ds.residual = float(resid[i])
+            ds._mode_field = X[i].cpu().numpy()
+        data.info["inner_iterations"] = list(arn.inner_iters)
+        data.info["sigma"] = sigma
         self.data = data
         return data

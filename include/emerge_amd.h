@@ -110,6 +110,17 @@ int emg_csr_find(int64_t m, const int32_t *d_rowptr, const int32_t *d_cols, cons
 /* d_K[d_pos[i]] += (gre + j gim) * d_v[i] for i < m (complex128, atomic). */
 int emg_scatter_add(int64_t m, const int32_t *d_pos, const double *d_v, double gre, double gim, double *d_K, void *stream);
 
+/* complex128 vector primitives (used by the shift-invert Arnoldi driver that
+ * replaces scipy eigs/eigsh, fem/solver.py:343-357 and the deprecated 3D call
+ * at fem/physics/edm/emfreq3d.py:838-848).  V holds nv vectors of n entries
+ * back to back; d_work is scratch of ceil(nv/8)*1024*16 + 1024 doubles.    */
+int emg_spmv(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_vals, const uint8_t *d_mask,
+             const double *d_x, double *d_y, void *stream);
+int emg_mdotc(int64_t n, int nv, const double *d_V, const double *d_w, double *d_h, double *d_work, void *stream);
+int emg_maxpy(int64_t n, int nv, const double *d_V, const double *d_h, double *d_w, double *d_norm, double *d_work,
+              void *stream);
+int emg_mcomb(int64_t n, int nv, const double *d_V, const double *d_y, double *d_x, void *stream);
+
 /* ------------------------------------------------------------------ boundary terms
  * Robin (port / absorbing boundary) triangle terms, replacing
  * ned2_tri_stiff_force and ned2_tri_stiff (fem/mth/tri.py:575-796) as called

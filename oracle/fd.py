@@ -260,3 +260,15 @@ class OracleProblem:
                 ap.active = False
             sols.append(fsol)
         return S, sols
+
+
+def oracle_eigenmodes(problem: OracleProblem, sigma: float, k: int, spurious_rel: float = 1e-3):
+    """Deprecated 3D eigen call of the reference (fem/physics/edm/emfreq3d.py:838-848):
+    eigs(A, k, M=B, sigma, which='LM') on the PEC-reduced E and B (ARPACK with a
+    direct shift-invert factorisation); lambda ~ 0 (gradient) modes removed."""
+    _, _, solve_ids, _ = problem.assemble_freq_matrix(1.0)
+    A = problem.E[solve_ids][:, solve_ids].tocsc()
+    B = problem.B[solve_ids][:, solve_ids].tocsc()
+    w = spla.eigs(A, k=k + 4, M=B, sigma=sigma, which="LM", return_eigenvectors=False)
+    w = w[np.abs(w) > spurious_rel * abs(sigma)]
+    return w[np.argsort(np.abs(w - sigma))][:k]

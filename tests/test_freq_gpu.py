@@ -65,6 +65,7 @@ def test_waveguide_sparams_match_direct(cuda, precision):
     data = phys.frequency_domain()
     for ds in data.datasets:
         for rec in ds.solves:
+            print(rec, flush=True)
             assert rec.converged and rec.residual <= 1e-10, rec
     Sg = data.S_array()
     So, _ = _oracle(mesh, phys).frequency_domain(freqs)
