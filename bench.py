@@ -51,6 +51,74 @@ def cpu_baseline(mesh, eps, sample_tets: int):
                       f"scipy coo->csr of E and B (single thread, as the reference), {dt:.2f} s"}
 
 
+def bench_sweep(args, dev):
+    """Frequency sweep of the WR-90 two-port section (20,160 tets, 135,878 DOFs).
+
+    Plain FP64 GMRES (complex128) and mixed-precision GMRES (complex64 cycles in
+    FP64 refinement), both to FP64 relative residual <= tol on every solve; the
+    host-core reference path (oracle assembly + MKL PARDISO + S-parameters) on
+    one frequency of the same problem."""
+    import torch
+    from emerge_amd import meshgen
+    from emerge_amd.bc import CoordinateSystem, RectangularWaveguide
+    from emerge_amd.physics import Electrodynamics3D
+    a, b, L = 22.86e-3, 10.16e-3, 50e-3
+    mesh = meshgen.waveguide_wr90()
+    cs1 = CoordinateSystem((1, 0, 0), (0, 1, 0), (0, 0, 1), (a / 2, b / 2, 0))
+    cs2 = CoordinateSystem((-1, 0, 0), (0, 1, 0), (0, 0, -1), (a / 2, b / 2, L))
+    freqs = list(np.linspace(9e9, 11e9, args.sweep_freqs))
+    out = {"workload": "WR-90 section 22.86x10.16x50 mm, 15x7x32 Kuhn hexes (20,160 tets), 2 waveguide ports",
+           "freqs": len(freqs), "tol": args.sweep_tol}
+    S_ref = None
+    for prec in (64, 32):
+        phys = Electrodynamics3D(mesh)
+        phys.assign(RectangularWaveguide([meshgen.ZMIN], 1, cs=cs1, dims=(a, b)),
+                    RectangularWaveguide([meshgen.ZMAX], 2, cs=cs2, dims=(a, b)))
+        phys.set_frequency(freqs)
+        st = phys.solveroutine
+        st.precision, st.tol, st.store_fields = prec, args.sweep_tol, False
+        phys.problem()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        data = phys.frequency_domain()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        recs = [r for ds in data.datasets for r in ds.solves]
+        key = "plain_fp64" if prec == 64 else "mixed_c64_fp64"
+        S = data.S_array()
+        out[key] = {"freq_pts_per_s": len(freqs) / dt, "seconds": dt,
+                    "iterations_total": int(sum(r.iterations for r in recs)),
+                    "max_fp64_residual": float(max(r.residual for r in recs)),
+                    "all_converged": bool(all(r.converged for r in recs))}
+        if S_ref is None:
+            S_ref = S
+        else:
+            out[key]["max_abs_dS_vs_plain"] = float(np.abs(S - S_ref).max())
+    out["speedup_mixed_vs_plain"] = out["mixed_c64_fp64"]["freq_pts_per_s"] / out["plain_fp64"]["freq_pts_per_s"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_sweep_baseline(mesh, phys, freqs[len(freqs) // 2])
+        out["speedup_mixed_vs_cpu"] = out["mixed_c64_fp64"]["freq_pts_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
+def cpu_sweep_baseline(mesh, phys, freq):
+    """The reference CPU path for one frequency of the same problem: oracle
+    (C/OpenMP) assembly, MKL PARDISO on the real block system (the reference's
+    x86 default solver), both port solves and S-parameters -- host cores."""
+    from oracle import fd
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    er, ur = phys.material_tensors()
+    t0 = time.perf_counter()
+    prob = fd.OracleProblem(mesh, er, ur, phys.boundary_conditions, solver="pardiso", threads=cores)
+    prob.frequency_domain([freq])
+    prob.release()
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "freq-pts/s", "cores": cores, "kind": "port",
+            "sample": f"one frequency of the same 135,878-DOF WR-90 problem: oracle C/OpenMP assembly, "
+                      f"MKL PARDISO factorisation of the real 2N block system ({cores} threads), "
+                      f"2 port solves, S-parameters: {dt:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +128,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-face-mass", action="store_true")
     ap.add_argument("--two-phase", action="store_true", help="time the per-tet-block fallback kernels")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the frequency-sweep measurement")
+    ap.add_argument("--sweep-freqs", type=int, default=5)
+    ap.add_argument("--sweep-tol", type=float, default=1e-8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device-to-device copy (reference for PMC byte counters)")
     args = ap.parse_args()
@@ -142,6 +213,10 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, eps, args.cpu_sample)
+    if not args.no_sweep:
+        del E, B, pat
+        torch.cuda.empty_cache()
+        out["sweep"] = bench_sweep(args, dev)
     print(json.dumps(out), flush=True)
 
 

@@ -129,3 +129,31 @@ def waveguide_wr90(nx=15, ny=7, nz=32, seed=0, length=50e-3):
     """Air-filled WR-90 section 22.86 x 10.16 x `length` mm, ports on z faces."""
     size = (22.86e-3, 10.16e-3, length)
     return structured_box(nx, ny, nz, size, jitter=0.05, seed=seed)
+
+
+POST_TAG = 7
+
+
+def posts_box(nx=80, ny=40, nz=16, seed=2, size=(40e-3, 20e-3, 10e-3), radius=1.5e-3, height=8e-3,
+              centers_x=(8e-3, 16e-3, 24e-3, 32e-3)):
+    """Sweep workload: box with PEC cylindrical posts on the centre line.
+
+    The posts (radius, height from z = 0) are carved out by tet centroid, so
+    their staircase surfaces become exterior faces (PEC by default); they carry
+    face tag POST_TAG.  80 x 40 x 16 hexes give 307,200 tets before carving.
+    """
+    yc = size[1] / 2
+
+    def keep(c):
+        inside = np.zeros(c.shape[1], dtype=bool)
+        for xc in centers_x:
+            inside |= ((c[0] - xc) ** 2 + (c[1] - yc) ** 2 < radius ** 2) & (c[2] < height)
+        return ~inside
+
+    mesh = structured_box(nx, ny, nz, size=size, jitter=0.05, seed=seed, keep=keep)
+    # tag the remaining exterior faces (the post walls)
+    bnd = mesh.boundary_triangles()
+    tagged = np.unique(np.concatenate(list(mesh.face_tags.values()))) if mesh.face_tags else np.zeros(0, int)
+    post = np.setdiff1d(bnd, tagged)
+    mesh.face_tags[POST_TAG] = post
+    return mesh

@@ -74,8 +74,10 @@ class OracleProblem:
     with OracleMesh(order_seed) and maps tagged triangles into it.
     """
 
-    def __init__(self, mesh, er, ur, bcs, order_seed=None, E=None, B=None):
+    def __init__(self, mesh, er, ur, bcs, order_seed=None, E=None, B=None, solver="superlu", threads=None):
         self.src = mesh
+        self.solver = solver
+        self.threads = threads
         self.om = O.OracleMesh(mesh.nodes, mesh.tets, order_seed=order_seed)
         self.field = O.OracleNedelec2(self.om)
         self.er, self.ur = er, ur
@@ -138,18 +140,38 @@ class OracleProblem:
 
     # fem/solver.py:405-469 (direct solve on the PEC-reduced system) with the
     # reference's SuperLU settings (solver.py:243-275: MMD_AT_PLUS_A ordering,
-    # diag_pivot_thresh 0.01, SymmetricMode); the factorisation of the last
-    # matrix is reused for the second port like reuse_factorization.
+    # diag_pivot_thresh 0.01, SymmetricMode) or, with solver="pardiso", its
+    # x86 default PARDISO on the real block system (solver.py:294-309); the
+    # factorisation of the last matrix is reused for the second port like
+    # reuse_factorization.
     def solve(self, K, b, solve_ids):
         x = np.zeros(K.shape[0], dtype=np.complex128)
         key = id(K)
+        n = len(solve_ids)
         if getattr(self, "_lu_key", None) != key:
-            A = K[solve_ids][:, solve_ids].tocsc()
-            self._lu = spla.splu(A, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.01,
-                                 options=dict(SymmetricMode=True))
+            self.release()
+            A = K[solve_ids][:, solve_ids]
+            if self.solver == "pardiso":
+                from . import pardiso
+                Ab, _ = pardiso.complex_to_real_block(A.tocsr(), np.zeros(n, dtype=np.complex128))
+                self._lu = pardiso.Pardiso(Ab, threads=self.threads)
+            else:
+                self._lu = spla.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.01,
+                                     options=dict(SymmetricMode=True))
             self._lu_key = key
-        x[solve_ids] = self._lu.solve(b[solve_ids])
+        bs = b[solve_ids]
+        if self.solver == "pardiso":
+            xr = self._lu.solve(np.hstack([bs.real, bs.imag]))
+            x[solve_ids] = xr[:n] + 1j * xr[n:]
+        else:
+            x[solve_ids] = self._lu.solve(bs)
         return x
+
+    def release(self):
+        lu = getattr(self, "_lu", None)
+        if lu is not None and self.solver == "pardiso":
+            lu.release()
+        self._lu, self._lu_key = None, None
 
     # fem/mth/tet.py:371-497 evaluated in the tets that hold the points
     def interpolate(self, x, pts, tet_ids=None):
