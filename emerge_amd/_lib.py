@@ -39,6 +39,9 @@ SIGNATURES: dict[str, tuple] = {
     "emg_mcomb": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P]),
     "emg_robin_blocks": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P, P]),
     "emg_interp_points": (ctypes.c_int, [I64, P, P, P, P, P, I64, ctypes.c_int, P, P, P]),
+    "emg_sweep_residual": (ctypes.c_int, [I64, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P]),
+    "emg_port_apply": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P, P, P, P]),
+    "emg_basis_combine": (ctypes.c_int, [I64, ctypes.c_int, P, ctypes.c_int, P, P, P]),
 }
 
 

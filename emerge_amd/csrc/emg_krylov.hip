@@ -8,9 +8,10 @@
 //     reduced matrix is ever formed (solver.py:430-435 builds one per call);
 //   * SpMV is HBM-bound: 16-lane sub-waves per CSR row, coalesced loads of
 //     columns and values, complex128 or complex64 values;
-//   * Gram-Schmidt is classical with one re-orthogonalisation (CGS2), as a
-//     tiled multi-dot and a fused multi-AXPY + norm, so each Arnoldi step
-//     streams the basis twice instead of j times;
+//   * Gram-Schmidt is classical, as a tiled multi-dot and a fused multi-AXPY
+//     + norm, so each Arnoldi step streams the basis twice instead of j
+//     times; a second pass runs only when the first cancelled most of the
+//     vector (loss-of-orthogonality test on the norms);
 //   * the preconditioner is the inverse of the 2x2 blocks coupling the two
 //     modes of each edge / face;
 //   * precision 32 runs GMRES in complex64 inside FP64 iterative refinement:
@@ -456,6 +457,21 @@ extern "C" int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const 
 // ---- one restarted GMRES solve of K u = rhs (element type R), right preconditioned.
 // On return u holds the solution; returns the iteration count through *iters and
 // the final residual estimate relative to |rhs| through *rel.
+// out[0] = sqrt(sum of nb partial sums of squares)
+__global__ void __launch_bounds__(RB) k_norm_from_partial(const double *__restrict__ partial, int nb,
+                                                          double *__restrict__ out) {
+    __shared__ double sh[RB];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) a += partial[i];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = RB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sqrt(sh[0]);
+}
+
 template <typename R>
 static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, R *u, double tol, int maxiter,
                       int *iters, double *rel, hipStream_t st) {
@@ -512,18 +528,26 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
             const int nv = j + 1;
             const dim3 grid(nb, (nv + DT - 1) / DT);
             double *h1 = s->h, *h2 = s->h + 2 * 512;
-            for (int pass = 0; pass < 2; pass++) {   // CGS2: project, subtract, repeat once
+            // Classical Gram-Schmidt with a second pass only when the first
+            // one cancelled more than 1 - 1/sqrt(2) of the vector (the
+            // Daniel-Gragg-Kaufman-Stewart criterion); ||w||^2 before the
+            // projection is ||w after||^2 + ||h||^2 for an orthonormal V.
+            std::vector<double> hh(2 * 2 * 512, 0.0);
+            double hn = 0.0;
+            for (int pass = 0; pass < 2; pass++) {
                 double *hp = pass ? h2 : h1;
                 k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial);
                 k_mdot_finish<<<nv, 256, 0, st>>>(nv, nb, s->partial, hp, 0);
                 k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, hp, vn, s->partial);
+                k_norm_from_partial<<<1, RB, 0, st>>>(s->partial, nb, s->nrm);
+                EMG_HIP(hipMemcpyAsync(hh.data() + pass * 2 * 512, hp, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, st));
+                EMG_HIP(hipMemcpyAsync(&hn, s->nrm, sizeof(double), hipMemcpyDeviceToHost, st));
+                EMG_HIP(hipStreamSynchronize(st));
+                double hsq = 0.0;
+                for (int i = 0; i < 2 * nv; i++) hsq += hh[pass * 2 * 512 + i] * hh[pass * 2 * 512 + i];
+                if (hn * hn >= 0.5 * (hn * hn + hsq)) break;
             }
-            std::vector<double> hh(2 * 2 * 512);
-            EMG_HIP(hipMemcpyAsync(hh.data(), s->h, 2 * 2 * 512 * sizeof(double), hipMemcpyDeviceToHost, st));
             k_scale_by_norm<R><<<nb, RB, 0, st>>>(n, s->partial, nb, vn, vn, s->nrm);
-            double hn;
-            EMG_HIP(hipMemcpyAsync(&hn, s->nrm, sizeof(double), hipMemcpyDeviceToHost, st));
-            EMG_HIP(hipStreamSynchronize(st));
             std::vector<double> hsum(2 * nv);
             for (int i = 0; i < 2 * nv; i++) hsum[i] = hh[i] + hh[2 * 512 + i];
             for (int i = 0; i < nv; i++) H[(size_t)i * m + j] = std::complex<double>(hsum[2 * i], hsum[2 * i + 1]);

@@ -131,16 +131,22 @@ def waveguide_wr90(nx=15, ny=7, nz=32, seed=0, length=50e-3):
     return structured_box(nx, ny, nz, size, jitter=0.05, seed=seed)
 
 
-POST_TAG = 7
+POST_TAG, PORT1_TAG, PORT2_TAG = 7, 8, 9
+PORT_WIDTH, PORT_HEIGHT = 2e-3, 3.75e-3
 
 
 def posts_box(nx=80, ny=40, nz=16, seed=2, size=(40e-3, 20e-3, 10e-3), radius=1.5e-3, height=8e-3,
-              centers_x=(8e-3, 16e-3, 24e-3, 32e-3)):
-    """Sweep workload: box with PEC cylindrical posts on the centre line.
+              centers_x=(8e-3, 16e-3, 24e-3, 32e-3), port_width=PORT_WIDTH, port_height=PORT_HEIGHT):
+    """Sweep workload: box with PEC cylindrical posts on the centre line and
+    two lumped-port rectangles on the x end faces.
 
     The posts (radius, height from z = 0) are carved out by tet centroid, so
     their staircase surfaces become exterior faces (PEC by default); they carry
-    face tag POST_TAG.  80 x 40 x 16 hexes give 307,200 tets before carving.
+    face tag POST_TAG.  On x = 0 and x = size[0] the triangles with centroid in
+    |y - yc| < port_width / 2, z < port_height (the strip from the floor up)
+    carry PORT1_TAG and PORT2_TAG; with the default 80 x 40 x 16 hexes these
+    edges fall on grid lines.  80 x 40 x 16 hexes give 307,200 tets before
+    carving, 298,447 after.
     """
     yc = size[1] / 2
 
@@ -156,4 +162,10 @@ def posts_box(nx=80, ny=40, nz=16, seed=2, size=(40e-3, 20e-3, 10e-3), radius=1.
     tagged = np.unique(np.concatenate(list(mesh.face_tags.values()))) if mesh.face_tags else np.zeros(0, int)
     post = np.setdiff1d(bnd, tagged)
     mesh.face_tags[POST_TAG] = post
+    for side, tag in ((XMIN, PORT1_TAG), (XMAX, PORT2_TAG)):
+        tri = mesh.face_tags[side]
+        c = mesh.nodes[:, mesh.tris[:, tri]].mean(axis=1)
+        inside = (np.abs(c[1] - yc) < port_width / 2) & (c[2] < port_height)
+        mesh.face_tags[tag] = tri[inside]
+        mesh.face_tags[side] = tri[~inside]
     return mesh

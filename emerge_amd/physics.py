@@ -26,10 +26,20 @@ class GPUSolveSettings:
     precision: 64 (complex128 GMRES) or 32 (complex64 GMRES in FP64 refinement)
     tol: FP64 relative residual target ||b - K x|| / ||b|| over free DOFs
     max_refine: cap on FP64 refinement cycles for precision 32 (0 = no cap besides maxiter)
+    sweep: "basis" (trial solutions from the span of earlier solutions, certified
+           by batched FP64 residuals, GMRES where a trial fails) or "plain"
+           (GMRES for every frequency and port, warm-started from the previous
+           frequency)
+    max_basis: cap on the number of stored basis vectors for sweep="basis"
     """
 
     def __init__(self, precision: int = 32, tol: float = 1e-8, restart: int = 150, maxiter: int = 60000,
-                 max_refine: int = 0, warm_start: bool = True, store_fields: bool = True):
+                 max_refine: int = 0, warm_start: bool = True, store_fields: bool = True, sweep: str = "basis",
+                 max_basis: int = 128):
+        if sweep not in ("basis", "plain"):
+            raise ValueError(f"sweep must be 'basis' or 'plain', not {sweep!r}")
+        self.sweep = sweep
+        self.max_basis = max_basis
         self.precision = precision
         self.tol = tol
         self.restart = restart
@@ -138,6 +148,33 @@ class Electrodynamics3D:
         er, ur = prob.er, prob.ur
         prev: dict[int, object] = {}
         t0 = time.perf_counter()
+        label = f"gmres-c{64 if st.precision == 32 else 128}"
+        if st.sweep == "basis":
+            from .sweep import BasisSweep
+            bs = BasisSweep(prob, solver, sampler, tol=st.tol, max_basis=st.max_basis, maxiter=st.maxiter,
+                            max_refine=st.max_refine)
+            res = bs.run(list(self.frequencies), label)
+            for fi, freq in enumerate(self.frequencies):
+                k0 = 2 * np.pi * freq / C0
+                ds = data.new(freq=freq, k0=k0)
+                ds.init_sp(nums)
+                ds.er = np.squeeze(er[0, 0, :])
+                ds.ur = np.squeeze(ur[0, 0, :])
+                for p in ports:
+                    ds.add_port_properties(p.port_number, p.mode_number, k0, p.get_beta(k0), p.Z0, p.power)
+                ys = [res[fi][i][0] for i in range(len(ports))]
+                ds.solves.extend(res[fi][i][1] for i in range(len(ports)))
+                S = sampler.s_matrix({p.port_number: bs.fields_of(ys[i]) for i, p in enumerate(ports)}, k0)
+                ds.Sp.arry[:, :] = S
+                if st.store_fields:
+                    for i, p in enumerate(ports):
+                        ds._fields[p.port_number] = bs.vector_of(ys[i]).cpu().numpy()
+                ds.set_field_vector()
+            torch.cuda.synchronize()
+            data.info["sweep_seconds"] = time.perf_counter() - t0
+            data.info["basis_size"] = bs.m
+            self.data = data
+            return data
         for freq in self.frequencies:
             k0 = 2 * np.pi * freq / C0
             ds = data.new(freq=freq, k0=k0)
@@ -154,7 +191,7 @@ class Electrodynamics3D:
                 x0 = prev.get(ap.port_number) if st.warm_start else None
                 x, info = solver.solve(b, x0=x0, tol=st.tol, maxiter=st.maxiter, max_refine=st.max_refine)
                 ds.solves.append(SolveRecord(ap.port_number, info.iterations, info.refinements, info.residual,
-                                             bool(info.converged), f"gmres-c{64 if st.precision == 32 else 128}"))
+                                             bool(info.converged), label))
                 prev[ap.port_number] = x
                 X.append(x)
             Xs = torch.stack(X)

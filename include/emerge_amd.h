@@ -145,6 +145,33 @@ int emg_interp_points(int64_t npts, const int32_t *d_ptet, const double *d_bary,
                       const int32_t *d_tets, const int32_t *d_tet_ent, int64_t n_ent, int nsol,
                       const double *d_x, double *d_E, void *stream);
 
+/* ------------------------------------------------------------------ batched sweep
+ * A block of nvec (frequency, port) pairs over the shared pattern; replaces the
+ * per-(frequency, port) assemble + solve loop of frequency_domain
+ * (fem/physics/edm/emfreq3d.py:658-694, assembler.py:312-388) where the sweep
+ * driver certifies trial solutions.  Multi-vectors are interleaved complex128:
+ * X[row * nvec + j].  nvec is 1, 2, 4, 8 or 16.
+ *
+ * emg_sweep_residual:
+ *   r_j = mask.(c_j Q[qidx_j] - (E - s_j B) x_j - Zp_j),  d_norms[2j] = ||r_j||^2,
+ *   d_norms[2j+1] = ||mask.c_j Q[qidx_j]||^2.  x_j must vanish on masked DOFs.
+ *   d_s (nvec) f64 = k0^2; d_Q (nq, n) c128 port sources for force scale 1;
+ *   d_c (nvec) c128 force scales; d_prow_of_row (n) int32 index into Zp or -1
+ *   (NULL together with d_Zp when there are no Robin rows); d_R (n, nvec) or
+ *   NULL; d_work >= 4096 * nvec * 2 doubles.                                    */
+int emg_sweep_residual(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_E,
+                       const double *d_B, const uint8_t *d_mask, int nvec, const double *d_s, const double *d_X,
+                       const double *d_Q, const int32_t *d_qidx, const double *d_c, const int32_t *d_prow_of_row,
+                       const double *d_Zp, double *d_R, double *d_norms, double *d_work, void *stream);
+/* Robin rows: Zp[i, j] = sum_e g[term_e, j] * val_e * X[col_e, j] over the
+ * entries d_ptr[i] .. d_ptr[i+1] of port row i (B_p(f) = gamma_p(f) P_p,
+ * assembler.py:382-383).  d_g (nterm, nvec) c128.                              */
+int emg_port_apply(int64_t nprow, int nvec, const int32_t *d_ptr, const int32_t *d_pcol, const int32_t *d_pterm,
+                   const double *d_pval, const double *d_g, const double *d_X, double *d_Zp, void *stream);
+/* X[row, j] = sum_i V[i, row] Y[j, i]: d_V (m, n) c128 basis, d_Y (nvec, m) c128,
+ * nvec * m <= 2048.                                                            */
+int emg_basis_combine(int64_t n, int m, const double *d_V, int nvec, const double *d_Y, double *d_X, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
