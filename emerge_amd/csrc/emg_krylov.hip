@@ -120,7 +120,9 @@ static constexpr int DT = 8;     // dot products per tile
 // partial[tile][blk][k] = sum over the block's elements of conj(V_{t*DT+k}) . w
 template <typename R>
 __global__ void __launch_bounds__(RB) k_mdot(int64_t n, int nv, const R *__restrict__ V, int64_t ldv,
-                                             const R *__restrict__ w, double *__restrict__ partial) {
+                                             const R *__restrict__ w, double *__restrict__ partial,
+                                             const double *__restrict__ run) {
+    if (run != nullptr && run[0] == 0.0) return;   // device-side skip (second Gram-Schmidt pass not needed)
     const int tile = blockIdx.y;
     const int i0 = tile * DT;
     double ar[DT], ai[DT];
@@ -158,9 +160,14 @@ __global__ void __launch_bounds__(RB) k_mdot(int64_t n, int nv, const R *__restr
 }
 
 // h[i] (+)= sum over blocks of the partials (complex, i < nv); one block per value
-__global__ void k_mdot_finish(int nv, int nblk, const double *__restrict__ partial, double *__restrict__ h, int accumulate) {
+__global__ void k_mdot_finish(int nv, int nblk, const double *__restrict__ partial, double *__restrict__ h, int accumulate,
+                              const double *__restrict__ run) {
     const int i = blockIdx.x;
     if (i >= nv) return;
+    if (run != nullptr && run[0] == 0.0) {
+        if (threadIdx.x == 0 && !accumulate) { h[2 * i] = 0.0; h[2 * i + 1] = 0.0; }
+        return;
+    }
     const int tile = i / DT, k = i % DT;
     double r = 0, im = 0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
@@ -187,7 +194,8 @@ __global__ void k_mdot_finish(int nv, int nblk, const double *__restrict__ parti
 template <typename R>
 __global__ void __launch_bounds__(RB) k_maxpy_norm(int64_t n, int nv, const R *__restrict__ V, int64_t ldv,
                                                    const double *__restrict__ h, R *__restrict__ w,
-                                                   double *__restrict__ partial) {
+                                                   double *__restrict__ partial, const double *__restrict__ run) {
+    if (run != nullptr && run[0] == 0.0) return;
     __shared__ double hs[2 * 512];
     for (int i = threadIdx.x; i < 2 * nv; i += RB) hs[i] = h[i];
     __syncthreads();
@@ -402,14 +410,54 @@ struct emg_solver {
     double *w = nullptr, *z = nullptr, *r = nullptr, *d = nullptr;
     double *partial = nullptr, *h = nullptr, *nrm = nullptr;
     int nblk = 0;
+    // Arnoldi steps run from captured HIP graphs (chunks of CHUNK_IT steps)
+    // with the Hessenberg column, Givens rotations and residual estimate
+    // updated on the device; the host reads the estimates once per chunk.
+    double *arn = nullptr;        // device Arnoldi state (see arn_off)
+    double *pn = nullptr;         // per-block partial norms of the new basis vector
+    double *flag = nullptr;       // [0]: 1 = run the second Gram-Schmidt pass
+    double *res_host = nullptr;   // pinned copy of the residual estimates
+    hipStream_t cap = nullptr;    // capture stream (graphs are launched on the caller's stream)
+    struct graphs {
+        std::vector<hipGraphExec_t> exec;
+        const void *vals = nullptr, *inv = nullptr, *mask = nullptr;
+    } gs[2];                      // [0] complex128 steps, [1] complex64 steps
 };
 
 static void free_solver(emg_solver *s) {
     if (!s) return;
-    void *ps[] = {s->vals32, s->inv64, s->inv32, s->V, s->w, s->z, s->r, s->d, s->partial, s->h, s->nrm};
+    for (auto &g : s->gs)
+        for (hipGraphExec_t e : g.exec) (void)hipGraphExecDestroy(e);
+    void *ps[] = {s->vals32, s->inv64, s->inv32, s->V, s->w, s->z, s->r, s->d, s->partial, s->h, s->nrm,
+                  s->arn, s->pn, s->flag};
     for (void *p : ps) (void)hipFree(p);
+    if (s->res_host) (void)hipHostFree(s->res_host);
+    if (s->cap) (void)hipStreamDestroy(s->cap);
     delete s;
 }
+
+// Blocks per tile of the multi-dot: about 2048 blocks in total, so each block
+// streams many rows and the per-block reduction is amortised.
+static int mdot_blocks(int64_t n, int nv) {
+    const int ntile = (nv + DT - 1) / DT;
+    const int64_t cap = std::max<int64_t>(64, 2048 / ntile);
+    return (int)std::max<int64_t>(1, std::min<int64_t>({(n + RB - 1) / RB, 1024, cap}));
+}
+
+// Offsets (in doubles) into the device Arnoldi state for restart length m.
+struct arn_off {
+    int64_t H, cs, sn, g, res, inv, total;
+    explicit arn_off(int m) {
+        H = 0;                                  // column j at 2 * j * (m + 1), rows 0..m
+        cs = 2 * (int64_t)m * (m + 1);
+        sn = cs + 2 * m;
+        g = sn + 2 * m;                         // m + 1 complex
+        res = g + 2 * (m + 1);                  // |g[j+1]| after step j
+        inv = res + m;                          // 1 / ||new vector|| of the last step
+        total = inv + 8;
+    }
+};
+static constexpr int CHUNK_IT = 10;   // Arnoldi steps per captured graph
 
 extern "C" int emg_solver_create(int64_t n, int64_t nnz, const int32_t *d_rowptr, const int32_t *d_cols, int restart,
                                  int precision, emg_solver **out) {
@@ -431,6 +479,11 @@ extern "C" int emg_solver_create(int64_t n, int64_t nnz, const int32_t *d_rowptr
     A((void **)&s->h, 3 * 2 * 512 * sizeof(double));   // h pass 1 | h pass 2 | y
     A((void **)&s->nrm, 4 * sizeof(double));
     A((void **)&s->inv64, s->n_ent * 4 * 16);
+    A((void **)&s->arn, arn_off(restart).total * sizeof(double));
+    A((void **)&s->pn, 1024 * sizeof(double));
+    A((void **)&s->flag, 8 * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&s->res_host, (restart + 8) * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking);
     if (precision == 32) { A((void **)&s->inv32, s->n_ent * 4 * 8); A((void **)&s->vals32, nnz * 8); }
     if (e != hipSuccess) {
         emg::set_error(std::string("emg_solver_create: ") + hipGetErrorString(e));
@@ -457,19 +510,151 @@ extern "C" int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const 
 // ---- one restarted GMRES solve of K u = rhs (element type R), right preconditioned.
 // On return u holds the solution; returns the iteration count through *iters and
 // the final residual estimate relative to |rhs| through *rel.
-// out[0] = sqrt(sum of nb partial sums of squares)
-__global__ void __launch_bounds__(RB) k_norm_from_partial(const double *__restrict__ partial, int nb,
-                                                          double *__restrict__ out) {
-    __shared__ double sh[RB];
-    double a = 0.0;
-    for (int i = threadIdx.x; i < nb; i += RB) a += partial[i];
-    sh[threadIdx.x] = a;
+// ---------------------------------------------------------------- device Arnoldi bookkeeping
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     __syncthreads();
-    for (int o = RB / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += sh[q];
+    return t;
+}
+
+// g = (beta, 0, ...), residual estimates cleared; beta = *nrm.
+__global__ void __launch_bounds__(RB) k_arn_init(int m, const double *__restrict__ nrm, double *__restrict__ arn,
+                                                 arn_off o) {
+    for (int i = threadIdx.x; i < 2 * (m + 1); i += RB) arn[o.g + i] = (i == 0) ? nrm[0] : 0.0;
+    for (int i = threadIdx.x; i < m; i += RB) arn[o.res + i] = 0.0;
+}
+
+// Second Gram-Schmidt pass only if the first removed more than half of the
+// vector's squared norm: ||w before||^2 = ||w after||^2 + ||h||^2.
+__global__ void __launch_bounds__(RB) k_arn_check(int nb, const double *__restrict__ pn, const double *__restrict__ h,
+                                                  int nv, double *__restrict__ flag) {
+    __shared__ double sh[RB / 64];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) a += pn[i];
+    for (int i = threadIdx.x; i < 2 * nv; i += RB) b += h[i] * h[i];
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) flag[0] = (a < 0.5 * (a + b)) ? 1.0 : 0.0;
+}
+
+// Hessenberg column j = h1 + h2 with ||new vector||, previous rotations, the
+// new rotation, g update, residual estimate |g[j+1]| and 1/||new vector||.
+// The column and the rotations are staged in LDS by the whole block; the
+// rotation chain itself is sequential and runs in registers of one lane.
+__global__ void __launch_bounds__(RB) k_arn_givens(int j, int m, int nb, const double *__restrict__ pn,
+                                                   const double *__restrict__ h1, const double *__restrict__ h2,
+                                                   double *__restrict__ arn, arn_off o) {
+    __shared__ double sh[RB / 64];
+    __shared__ double2 col[512], rot[2 * 512];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RB) a += pn[i];
+    for (int i = threadIdx.x; i <= j; i += RB) col[i] = make_double2(h1[2 * i] + h2[2 * i], h1[2 * i + 1] + h2[2 * i + 1]);
+    const double *cs = arn + o.cs, *sn = arn + o.sn;
+    for (int i = threadIdx.x; i < j; i += RB) {
+        rot[2 * i] = make_double2(cs[2 * i], cs[2 * i + 1]);
+        rot[2 * i + 1] = make_double2(sn[2 * i], sn[2 * i + 1]);
     }
-    if (threadIdx.x == 0) out[0] = sqrt(sh[0]);
+    a = block_sum(a, sh);          // contains __syncthreads: col / rot are visible after it
+    if (threadIdx.x != 0) return;
+    const double hn = sqrt(a);
+    // carry = current entry i of the column after rotations 0..i-1
+    double xr = col[0].x, xi = col[0].y;
+    for (int i = 0; i < j; i++) {
+        const double2 c = rot[2 * i], sv = rot[2 * i + 1], y = col[i + 1];
+        // col[i] = conj(c) x + conj(s) y ; carry = -s x + c y
+        col[i] = make_double2(c.x * xr + c.y * xi + sv.x * y.x + sv.y * y.y,
+                              c.x * xi - c.y * xr + sv.x * y.y - sv.y * y.x);
+        const double nr = -(sv.x * xr - sv.y * xi) + (c.x * y.x - c.y * y.y);
+        const double ni = -(sv.x * xi + sv.y * xr) + (c.x * y.y + c.y * y.x);
+        xr = nr;
+        xi = ni;
+    }
+    // new rotation from (x, hn)
+    const double den = sqrt(xr * xr + xi * xi + hn * hn);
+    double cr = 1.0, ci = 0.0, sr = 0.0, si = 0.0;
+    if (den > 0.0) { cr = xr / den; ci = xi / den; sr = hn / den; }
+    double *Hc = arn + o.H + 2 * (int64_t)j * (m + 1);
+    for (int i = 0; i < j; i++) { Hc[2 * i] = col[i].x; Hc[2 * i + 1] = col[i].y; }
+    Hc[2 * j] = cr * xr + ci * xi + sr * hn;          // conj(c) x + conj(s) hn
+    Hc[2 * j + 1] = cr * xi - ci * xr;
+    Hc[2 * j + 2] = 0.0;
+    Hc[2 * j + 3] = 0.0;
+    double *csw = arn + o.cs, *snw = arn + o.sn, *g = arn + o.g;
+    csw[2 * j] = cr; csw[2 * j + 1] = ci;
+    snw[2 * j] = sr; snw[2 * j + 1] = si;
+    const double gr = g[2 * j], gi = g[2 * j + 1];
+    g[2 * j + 2] = -(sr * gr - si * gi);
+    g[2 * j + 3] = -(sr * gi + si * gr);
+    g[2 * j] = cr * gr + ci * gi;
+    g[2 * j + 1] = cr * gi - ci * gr;
+    arn[o.res + j] = sqrt(g[2 * j + 2] * g[2 * j + 2] + g[2 * j + 3] * g[2 * j + 3]);
+    arn[o.inv] = hn > 0.0 ? 1.0 / hn : 0.0;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(RB) k_scale_inv(int64_t n, const double *__restrict__ inv, R *__restrict__ v) {
+    const double s = inv[0];
+    for (int64_t e = blockIdx.x * (int64_t)RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) {
+        const cx<R> a = ld(v, e);
+        st(v, e, cx<R>((R)(a.re * s), (R)(a.im * s)));
+    }
+}
+
+// One Arnoldi step j: v_{j+1} = K M^{-1} v_j, CGS (second pass decided on the
+// device), Hessenberg/Givens update, normalisation.  Only kernel launches, so
+// it can be captured.
+template <typename R>
+static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipStream_t st) {
+    const int64_t n = s->n;
+    const int m = s->restart, nb = s->nblk, nv = j + 1;
+    R *V = reinterpret_cast<R *>(s->V);
+    R *z = reinterpret_cast<R *>(s->z);
+    R *vj = V + (size_t)j * n * 2, *vn = V + (size_t)(j + 1) * n * 2;
+    double *h1 = s->h, *h2 = s->h + 2 * 512;
+    const arn_off o(m);
+    const int nbm = mdot_blocks(n, nv);
+    const dim3 grid(nbm, (nv + DT - 1) / DT);
+    k_bj_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, vj, z);
+    k_spmv<R><<<(unsigned)((n * SUB + 255) / 256), 256, 0, st>>>(n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn);
+    k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, nullptr);
+    k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, h1, 0, nullptr);
+    k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, h1, vn, s->pn, nullptr);
+    k_arn_check<<<1, RB, 0, st>>>(nb, s->pn, h1, nv, s->flag);
+    k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, s->flag);
+    k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, h2, 0, s->flag);
+    k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, h2, vn, s->pn, s->flag);
+    k_arn_givens<<<1, RB, 0, st>>>(j, m, nb, s->pn, h1, h2, s->arn, o);
+    k_scale_inv<R><<<nb, RB, 0, st>>>(n, s->arn + o.inv, vn);
+}
+
+// Capture the m steps of a restart cycle as graphs of CHUNK_IT steps (once per
+// solver, precision and matrix/preconditioner/mask buffers).
+template <typename R>
+static int ensure_graphs(emg_solver *s, const R *vals, const R *inv) {
+    auto &g = s->gs[sizeof(R) == 8 ? 0 : 1];
+    if (!g.exec.empty() && g.vals == vals && g.inv == inv && g.mask == s->mask) return 0;
+    for (hipGraphExec_t e : g.exec) (void)hipGraphExecDestroy(e);
+    g.exec.clear();
+    const int m = s->restart;
+    for (int c0 = 0; c0 < m; c0 += CHUNK_IT) {
+        EMG_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
+        for (int j = c0; j < std::min(m, c0 + CHUNK_IT); j++) enqueue_step<R>(s, vals, inv, j, s->cap);
+        hipGraph_t graph = nullptr;
+        EMG_HIP(hipStreamEndCapture(s->cap, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        EMG_HIP(e);
+        g.exec.push_back(exec);
+    }
+    g.vals = vals;
+    g.inv = inv;
+    g.mask = s->mask;
+    return 0;
 }
 
 template <typename R>
@@ -478,18 +663,12 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     const int64_t n = s->n;
     const int m = s->restart;
     const int nb = s->nblk;
+    const arn_off o(m);
     R *V = reinterpret_cast<R *>(s->V);
     R *w = reinterpret_cast<R *>(s->w);
     R *z = reinterpret_cast<R *>(s->z);
     const unsigned spmv_grid = (unsigned)((n * SUB + 255) / 256);
-    auto spmv = [&](const R *x, const R *b, R *y) {
-        k_spmv<R><<<spmv_grid, 256, 0, st>>>(n, s->rowptr, s->cols, vals, s->mask, x, b, y);
-    };
     auto prec = [&](const R *x, R *y) { k_bj_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, x, y); };
-    auto norm_scale = [&](const R *x, R *y, double *out) {
-        k_norm2<R><<<nb, RB, 0, st>>>(n, x, s->partial);
-        k_scale_by_norm<R><<<nb, RB, 0, st>>>(n, s->partial, nb, x, y, out);
-    };
     double bnorm = 0;
     {
         k_norm2<R><<<nb, RB, 0, st>>>(n, rhs, s->partial);
@@ -505,75 +684,48 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
         EMG_HIP(hipMemsetAsync(u, 0, n * 2 * sizeof(R), st));
         return 0;
     }
-    std::vector<std::complex<double>> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
-    std::vector<double> hbuf(2 * (m + 2));
+    if (int rc = ensure_graphs<R>(s, vals, inv)) return rc;
+    const auto &graphs = s->gs[sizeof(R) == 8 ? 0 : 1].exec;
+    std::vector<double> Hh, gh(2 * (m + 1));
+    std::vector<std::complex<double>> y(m);
     int total = 0;
     double resid = 1.0;
     while (true) {
         // r = rhs - K u  ->  v0 = r / |r|
-        spmv(u, rhs, w);
-        norm_scale(w, V, s->nrm);
+        k_spmv<R><<<spmv_grid, 256, 0, st>>>(n, s->rowptr, s->cols, vals, s->mask, u, rhs, w);
+        k_norm2<R><<<nb, RB, 0, st>>>(n, w, s->partial);
+        k_scale_by_norm<R><<<nb, RB, 0, st>>>(n, s->partial, nb, w, V, s->nrm);
         double beta;
         EMG_HIP(hipMemcpyAsync(&beta, s->nrm, sizeof(double), hipMemcpyDeviceToHost, st));
         EMG_HIP(hipStreamSynchronize(st));
         resid = beta / bnorm;
         if (resid <= tol || total >= maxiter || !(beta > 0) || !std::isfinite(beta)) break;
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = beta;
-        int j = 0;
-        for (; j < m && total < maxiter; j++, total++) {
-            R *vj = V + (size_t)j * n * 2, *vn = V + (size_t)(j + 1) * n * 2;
-            prec(vj, z);
-            spmv(z, nullptr, vn);
-            const int nv = j + 1;
-            const dim3 grid(nb, (nv + DT - 1) / DT);
-            double *h1 = s->h, *h2 = s->h + 2 * 512;
-            // Classical Gram-Schmidt with a second pass only when the first
-            // one cancelled more than 1 - 1/sqrt(2) of the vector (the
-            // Daniel-Gragg-Kaufman-Stewart criterion); ||w||^2 before the
-            // projection is ||w after||^2 + ||h||^2 for an orthonormal V.
-            std::vector<double> hh(2 * 2 * 512, 0.0);
-            double hn = 0.0;
-            for (int pass = 0; pass < 2; pass++) {
-                double *hp = pass ? h2 : h1;
-                k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial);
-                k_mdot_finish<<<nv, 256, 0, st>>>(nv, nb, s->partial, hp, 0);
-                k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, hp, vn, s->partial);
-                k_norm_from_partial<<<1, RB, 0, st>>>(s->partial, nb, s->nrm);
-                EMG_HIP(hipMemcpyAsync(hh.data() + pass * 2 * 512, hp, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, st));
-                EMG_HIP(hipMemcpyAsync(&hn, s->nrm, sizeof(double), hipMemcpyDeviceToHost, st));
-                EMG_HIP(hipStreamSynchronize(st));
-                double hsq = 0.0;
-                for (int i = 0; i < 2 * nv; i++) hsq += hh[pass * 2 * 512 + i] * hh[pass * 2 * 512 + i];
-                if (hn * hn >= 0.5 * (hn * hn + hsq)) break;
+        k_arn_init<<<1, RB, 0, st>>>(m, s->nrm, s->arn, o);
+        int k = 0;
+        bool done = false;
+        for (int c = 0; c < (int)graphs.size() && !done; c++) {
+            const int j0 = c * CHUNK_IT, j1 = std::min(m, j0 + CHUNK_IT);
+            EMG_HIP(hipGraphLaunch(graphs[c], st));
+            EMG_HIP(hipMemcpyAsync(s->res_host + j0, s->arn + o.res + j0, (j1 - j0) * sizeof(double),
+                                   hipMemcpyDeviceToHost, st));
+            EMG_HIP(hipStreamSynchronize(st));
+            for (int j = j0; j < j1; j++) {
+                k = j + 1;
+                resid = s->res_host[j] / bnorm;
+                if (!std::isfinite(resid) || resid <= tol || total + k >= maxiter) { done = true; break; }
             }
-            k_scale_by_norm<R><<<nb, RB, 0, st>>>(n, s->partial, nb, vn, vn, s->nrm);
-            std::vector<double> hsum(2 * nv);
-            for (int i = 0; i < 2 * nv; i++) hsum[i] = hh[i] + hh[2 * 512 + i];
-            for (int i = 0; i < nv; i++) H[(size_t)i * m + j] = std::complex<double>(hsum[2 * i], hsum[2 * i + 1]);
-            H[(size_t)(j + 1) * m + j] = hn;
-            for (int i = 0; i < j; i++) {   // previous rotations
-                const auto a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
-                H[(size_t)i * m + j] = std::conj(cs[i]) * a + std::conj(sn[i]) * b;
-                H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
-            }
-            const auto a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
-            const double den = std::sqrt(std::norm(a) + std::norm(b));
-            if (den == 0) { cs[j] = 1; sn[j] = 0; }
-            else { cs[j] = a / den; sn[j] = b / den; }
-            H[(size_t)j * m + j] = std::conj(cs[j]) * a + std::conj(sn[j]) * b;
-            H[(size_t)(j + 1) * m + j] = 0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = std::conj(cs[j]) * g[j];
-            resid = std::abs(g[j + 1]) / bnorm;
-            if (!std::isfinite(resid) || resid <= tol || hn == 0) { j++; total++; break; }
         }
-        // y = H^{-1} g ; u += M^{-1} V y
-        const int k = j;
+        total += k;
+        // y = R^{-1} g (upper triangular after the rotations) ; u += M^{-1} V y
+        Hh.resize(2 * (size_t)k * (m + 1));
+        EMG_HIP(hipMemcpyAsync(Hh.data(), s->arn + o.H, Hh.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        EMG_HIP(hipMemcpyAsync(gh.data(), s->arn + o.g, 2 * (m + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+        EMG_HIP(hipStreamSynchronize(st));
+        auto Hc = [&](int i, int q) { return std::complex<double>(Hh[2 * ((size_t)q * (m + 1) + i)], Hh[2 * ((size_t)q * (m + 1) + i) + 1]); };
         for (int i = k - 1; i >= 0; i--) {
-            std::complex<double> acc = g[i];
-            for (int q = i + 1; q < k; q++) acc -= H[(size_t)i * m + q] * y[q];
-            y[i] = acc / H[(size_t)i * m + i];
+            std::complex<double> acc(gh[2 * i], gh[2 * i + 1]);
+            for (int q = i + 1; q < k; q++) acc -= Hc(i, q) * y[q];
+            y[i] = acc / Hc(i, i);
         }
         std::vector<double> yb(2 * k);
         for (int i = 0; i < k; i++) { yb[2 * i] = y[i].real(); yb[2 * i + 1] = y[i].imag(); }
@@ -710,9 +862,9 @@ extern "C" int emg_mdotc(int64_t n, int nv, const double *d_V, const double *d_w
                          void *stream) {
     EMG_CHECK(d_V && d_w && d_h && d_work && nv > 0 && nv <= 512, "emg_mdotc: bad argument");
     hipStream_t st = (hipStream_t)stream;
-    const int nb = nblk_for(n);
-    k_mdot<double><<<dim3(nb, (nv + DT - 1) / DT), RB, 0, st>>>(n, nv, d_V, n, d_w, d_work);
-    k_mdot_finish<<<nv, 256, 0, st>>>(nv, nb, d_work, d_h, 0);
+    const int nbm = mdot_blocks(n, nv);
+    k_mdot<double><<<dim3(nbm, (nv + DT - 1) / DT), RB, 0, st>>>(n, nv, d_V, n, d_w, d_work, nullptr);
+    k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, d_work, d_h, 0, nullptr);
     EMG_LAUNCH_CHECK();
     return 0;
 }
@@ -723,7 +875,7 @@ extern "C" int emg_maxpy(int64_t n, int nv, const double *d_V, const double *d_h
     EMG_CHECK(d_V && d_h && d_w && d_norm && d_work && nv >= 0 && nv <= 512, "emg_maxpy: bad argument");
     hipStream_t st = (hipStream_t)stream;
     const int nb = nblk_for(n);
-    k_maxpy_norm<double><<<nb, RB, 0, st>>>(n, nv, d_V, n, d_h, d_w, d_work);
+    k_maxpy_norm<double><<<nb, RB, 0, st>>>(n, nv, d_V, n, d_h, d_w, d_work, nullptr);
     k_scale_by_norm<double><<<1, RB, 0, st>>>(0, d_work, nb, d_w, d_w, d_norm);   // n = 0: norm only
     EMG_LAUNCH_CHECK();
     return 0;

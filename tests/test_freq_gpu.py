@@ -62,6 +62,7 @@ def test_waveguide_sparams_match_direct(cuda, precision):
     phys.set_frequency(freqs)
     phys.solveroutine.precision = precision
     phys.solveroutine.tol = 1e-10
+    phys.solveroutine.sweep = "plain"
     data = phys.frequency_domain()
     for ds in data.datasets:
         for rec in ds.solves:
