@@ -28,6 +28,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_assemble_tensor": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, P, P, P]),
     "emg_solver_create": (ctypes.c_int, [I64, I64, P, P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P)]),
     "emg_solver_set_matrix": (ctypes.c_int, [P, P, P, P]),
+    "emg_solver_set_gradient": (ctypes.c_int, [P, I64, P, P, P, P, P, P, P]),
     "emg_solver_solve": (ctypes.c_int, [P, P, P, F64, ctypes.c_int, ctypes.c_int, P, P]),
     "emg_solver_destroy": (None, [P]),
     "emg_affine_combine": (ctypes.c_int, [I64, P, P, F64, P, P]),

@@ -307,6 +307,96 @@ __device__ __forceinline__ int32_t find_col(const int32_t *cols, int32_t p0, int
 }
 
 // Build the inverse 2x2 blocks from K (complex128); PEC DOFs get identity.
+// ---------------------------------------------------------------- gradient correction
+static constexpr int G_ROW_MAX = 8;   // entries per row of G (edge rows 3, face rows 6)
+
+// dS[p] += g_ip * sum_j K_ij g_jp over the unmasked entries of row i, for the
+// P2 nodes p of row i of G: diag(G^T K G) with one atomic per entry of G.
+__global__ void __launch_bounds__(RB) k_gdiag(int64_t n, const int32_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ cols, const double *__restrict__ K,
+                                              const uint8_t *__restrict__ mask, const int32_t *__restrict__ grp,
+                                              const int32_t *__restrict__ gcol, const double *__restrict__ gval,
+                                              double *__restrict__ dS) {
+    const int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x;
+    if (i >= n || (mask && !mask[i])) return;
+    const int32_t q0 = grp[i], ni = min(grp[i + 1] - q0, G_ROW_MAX);
+    if (ni == 0) return;
+    int32_t pk[G_ROW_MAX];
+    double ar[G_ROW_MAX], ai[G_ROW_MAX];
+    for (int k = 0; k < G_ROW_MAX; k++) { pk[k] = k < ni ? gcol[q0 + k] : -1; ar[k] = 0.0; ai[k] = 0.0; }
+    for (int32_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+        const int32_t j = cols[e];
+        if (mask && !mask[j]) continue;
+        const int32_t r0 = grp[j], r1 = grp[j + 1];
+        if (r0 == r1) continue;
+        const double kr = K[2 * e], ki = K[2 * e + 1];
+        for (int32_t q = r0; q < r1; q++) {
+            const int32_t pj = gcol[q];
+            const double gj = gval[q];
+#pragma unroll
+            for (int k = 0; k < G_ROW_MAX; k++)
+                if (pk[k] == pj) { ar[k] += kr * gj; ai[k] += ki * gj; }
+        }
+    }
+    for (int k = 0; k < ni; k++) {
+        const double gi = gval[q0 + k];
+        atomicAdd(&dS[2 * pk[k]], gi * ar[k]);
+        atomicAdd(&dS[2 * pk[k] + 1], gi * ai[k]);
+    }
+}
+
+__global__ void __launch_bounds__(RB) k_dsinv(int64_t np2, const double *__restrict__ dS, double *__restrict__ inv64,
+                                              float *__restrict__ inv32) {
+    const int64_t p = blockIdx.x * (int64_t)RB + threadIdx.x;
+    if (p >= np2) return;
+    const double a = dS[2 * p], b = dS[2 * p + 1], d = a * a + b * b;
+    const double r = d > 0.0 ? a / d : 0.0, im = d > 0.0 ? -b / d : 0.0;
+    inv64[2 * p] = r;
+    inv64[2 * p + 1] = im;
+    if (inv32) { inv32[2 * p] = (float)r; inv32[2 * p + 1] = (float)im; }
+}
+
+// t[p] = dsinv[p] * sum over row p of G^T of g * r
+template <typename R>
+__global__ void __launch_bounds__(RB) k_gt_apply(int64_t np2, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                 const R *__restrict__ gv, const R *__restrict__ dsinv,
+                                                 const R *__restrict__ r, R *__restrict__ t) {
+    const int64_t p = blockIdx.x * (int64_t)RB + threadIdx.x;
+    if (p >= np2) return;
+    double ar = 0.0, ai = 0.0;
+    for (int32_t e = rp[p]; e < rp[p + 1]; e++) {
+        const cx<R> v = ld(r, ci[e]);
+        ar += (double)gv[e] * v.re;
+        ai += (double)gv[e] * v.im;
+    }
+    const cx<R> d = ld(dsinv, p);
+    st(t, p, cx<R>((R)(d.re * ar - d.im * ai), (R)(d.re * ai + d.im * ar)));
+}
+
+// z = BJ(r) + G t, both modes of entity e per thread
+template <typename R>
+__global__ void __launch_bounds__(RB) k_bj_grad_apply(int64_t n_ent, const R *__restrict__ inv, const R *__restrict__ r,
+                                                      const int32_t *__restrict__ grp, const int32_t *__restrict__ gcol,
+                                                      const R *__restrict__ gv, const R *__restrict__ t,
+                                                      R *__restrict__ z) {
+    const int64_t e = blockIdx.x * (int64_t)RB + threadIdx.x;
+    if (e >= n_ent) return;
+    const cx<R> a = ld(inv, 4 * e), b = ld(inv, 4 * e + 1), c = ld(inv, 4 * e + 2), d = ld(inv, 4 * e + 3);
+    const cx<R> r0 = ld(r, e), r1 = ld(r, e + n_ent);
+    cx<R> z0 = cadd(cmul(a, r0), cmul(b, r1));
+    cx<R> z1 = cadd(cmul(c, r0), cmul(d, r1));
+    for (int32_t q = grp[e]; q < grp[e + 1]; q++) {
+        const cx<R> tv = ld(t, gcol[q]);
+        z0 = cx<R>(z0.re + gv[q] * tv.re, z0.im + gv[q] * tv.im);
+    }
+    for (int32_t q = grp[e + n_ent]; q < grp[e + n_ent + 1]; q++) {
+        const cx<R> tv = ld(t, gcol[q]);
+        z1 = cx<R>(z1.re + gv[q] * tv.re, z1.im + gv[q] * tv.im);
+    }
+    st(z, e, z0);
+    st(z, e + n_ent, z1);
+}
+
 __global__ void __launch_bounds__(RB) k_bj_setup(int64_t n_ent, const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ cols, const double *__restrict__ vals,
                                                  const uint8_t *__restrict__ mask, double *__restrict__ inv64,
@@ -418,9 +508,17 @@ struct emg_solver {
     double *flag = nullptr;       // [0]: 1 = run the second Gram-Schmidt pass
     double *res_host = nullptr;   // pinned copy of the residual estimates
     hipStream_t cap = nullptr;    // capture stream (graphs are launched on the caller's stream)
+    // optional gradient correction of the smoother: G (n x np2), G^T (np2 x n)
+    int64_t np2 = 0;
+    const int32_t *g_rowptr = nullptr, *g_cols = nullptr, *gt_rowptr = nullptr, *gt_cols = nullptr;
+    const double *g_vals = nullptr, *gt_vals = nullptr;   // caller-owned
+    float *g_vals32 = nullptr, *gt_vals32 = nullptr;      // owned copies for complex64 steps
+    double *dS = nullptr, *dsinv64 = nullptr, *tg = nullptr;
+    float *dsinv32 = nullptr;
     struct graphs {
         std::vector<hipGraphExec_t> exec;
         const void *vals = nullptr, *inv = nullptr, *mask = nullptr;
+        int64_t np2 = 0;
     } gs[2];                      // [0] complex128 steps, [1] complex64 steps
 };
 
@@ -429,12 +527,28 @@ static void free_solver(emg_solver *s) {
     for (auto &g : s->gs)
         for (hipGraphExec_t e : g.exec) (void)hipGraphExecDestroy(e);
     void *ps[] = {s->vals32, s->inv64, s->inv32, s->V, s->w, s->z, s->r, s->d, s->partial, s->h, s->nrm,
-                  s->arn, s->pn, s->flag};
+                  s->arn, s->pn, s->flag, s->g_vals32, s->gt_vals32, s->dS, s->dsinv64, s->tg, s->dsinv32};
     for (void *p : ps) (void)hipFree(p);
     if (s->res_host) (void)hipHostFree(s->res_host);
     if (s->cap) (void)hipStreamDestroy(s->cap);
     delete s;
 }
+
+// The smoother: 2x2 block Jacobi, plus the gradient correction when set.
+template <typename R>
+static void apply_prec(emg_solver *s, const R *inv, const R *x, R *y, hipStream_t st) {
+    if (s->np2 > 0) {
+        const R *gv = sizeof(R) == 8 ? reinterpret_cast<const R *>(s->g_vals) : reinterpret_cast<const R *>(s->g_vals32);
+        const R *gtv = sizeof(R) == 8 ? reinterpret_cast<const R *>(s->gt_vals) : reinterpret_cast<const R *>(s->gt_vals32);
+        const R *dsi = sizeof(R) == 8 ? reinterpret_cast<const R *>(s->dsinv64) : reinterpret_cast<const R *>(s->dsinv32);
+        R *t = reinterpret_cast<R *>(s->tg);
+        k_gt_apply<R><<<emg::grid_for(s->np2, RB), RB, 0, st>>>(s->np2, s->gt_rowptr, s->gt_cols, gtv, dsi, x, t);
+        k_bj_grad_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, x, s->g_rowptr, s->g_cols, gv, t, y);
+    } else {
+        k_bj_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, x, y);
+    }
+}
+
 
 // Blocks per tile of the multi-dot: about 2048 blocks in total, so each block
 // streams many rows and the per-block reduction is amortised.
@@ -503,7 +617,55 @@ extern "C" int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const 
     s->mask = d_mask;
     k_bj_setup<<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, s->rowptr, s->cols, d_vals, d_mask, s->inv64, s->inv32);
     if (s->precision == 32) k_to_c64<<<1024, RB, 0, st>>>(s->nnz, d_vals, s->vals32);
+    if (s->np2 > 0) {
+        EMG_HIP(hipMemsetAsync(s->dS, 0, s->np2 * 16, st));
+        k_gdiag<<<emg::grid_for(s->n, RB), RB, 0, st>>>(s->n, s->rowptr, s->cols, d_vals, d_mask, s->g_rowptr,
+                                                         s->g_cols, s->g_vals, s->dS);
+        k_dsinv<<<emg::grid_for(s->np2, RB), RB, 0, st>>>(s->np2, s->dS, s->dsinv64, s->dsinv32);
+    }
     EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+__global__ void __launch_bounds__(RB) k_f64_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    for (int64_t e = blockIdx.x * (int64_t)RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) b[e] = (float)a[e];
+}
+
+extern "C" int emg_solver_set_gradient(emg_solver *s, int64_t np2, const int32_t *d_g_rowptr, const int32_t *d_g_cols,
+                                       const double *d_g_vals, const int32_t *d_gt_rowptr, const int32_t *d_gt_cols,
+                                       const double *d_gt_vals, void *stream) {
+    EMG_CHECK(s != nullptr && np2 >= 0, "emg_solver_set_gradient: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (np2 == 0) {
+        s->np2 = 0;
+        return 0;
+    }
+    EMG_CHECK(d_g_rowptr && d_g_cols && d_g_vals && d_gt_rowptr && d_gt_cols && d_gt_vals,
+              "emg_solver_set_gradient: null argument");
+    int32_t nnz_g = 0, nnz_gt = 0;
+    EMG_HIP(hipMemcpyAsync(&nnz_g, d_g_rowptr + s->n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    EMG_HIP(hipMemcpyAsync(&nnz_gt, d_gt_rowptr + np2, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    EMG_HIP(hipStreamSynchronize(st));
+    EMG_CHECK(nnz_g == nnz_gt && nnz_g > 0, "emg_solver_set_gradient: G and G^T entry counts differ");
+    EMG_CHECK(nnz_g <= G_ROW_MAX * s->n, "emg_solver_set_gradient: more than 8 entries per row of G");
+    void *old[] = {s->g_vals32, s->gt_vals32, s->dS, s->dsinv64, s->tg, s->dsinv32};
+    for (void *p : old) (void)hipFree(p);
+    s->g_vals32 = s->gt_vals32 = nullptr;
+    s->dS = s->dsinv64 = s->tg = nullptr;
+    s->dsinv32 = nullptr;
+    s->np2 = 0;
+    EMG_HIP(hipMalloc((void **)&s->g_vals32, (size_t)nnz_g * sizeof(float)));
+    EMG_HIP(hipMalloc((void **)&s->gt_vals32, (size_t)nnz_g * sizeof(float)));
+    EMG_HIP(hipMalloc((void **)&s->dS, (size_t)np2 * 16));
+    EMG_HIP(hipMalloc((void **)&s->dsinv64, (size_t)np2 * 16));
+    EMG_HIP(hipMalloc((void **)&s->dsinv32, (size_t)np2 * 8));
+    EMG_HIP(hipMalloc((void **)&s->tg, (size_t)np2 * 16));
+    k_f64_to_f32<<<1024, RB, 0, st>>>(nnz_g, d_g_vals, s->g_vals32);
+    k_f64_to_f32<<<1024, RB, 0, st>>>(nnz_g, d_gt_vals, s->gt_vals32);
+    EMG_LAUNCH_CHECK();
+    s->g_rowptr = d_g_rowptr; s->g_cols = d_g_cols; s->g_vals = d_g_vals;
+    s->gt_rowptr = d_gt_rowptr; s->gt_cols = d_gt_cols; s->gt_vals = d_gt_vals;
+    s->np2 = np2;
     return 0;
 }
 
@@ -618,7 +780,7 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
     const arn_off o(m);
     const int nbm = mdot_blocks(n, nv);
     const dim3 grid(nbm, (nv + DT - 1) / DT);
-    k_bj_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, vj, z);
+    apply_prec<R>(s, inv, vj, z, st);
     k_spmv<R><<<(unsigned)((n * SUB + 255) / 256), 256, 0, st>>>(n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn);
     k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, nullptr);
     k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, h1, 0, nullptr);
@@ -636,7 +798,7 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
 template <typename R>
 static int ensure_graphs(emg_solver *s, const R *vals, const R *inv) {
     auto &g = s->gs[sizeof(R) == 8 ? 0 : 1];
-    if (!g.exec.empty() && g.vals == vals && g.inv == inv && g.mask == s->mask) return 0;
+    if (!g.exec.empty() && g.vals == vals && g.inv == inv && g.mask == s->mask && g.np2 == s->np2) return 0;
     for (hipGraphExec_t e : g.exec) (void)hipGraphExecDestroy(e);
     g.exec.clear();
     const int m = s->restart;
@@ -654,6 +816,7 @@ static int ensure_graphs(emg_solver *s, const R *vals, const R *inv) {
     g.vals = vals;
     g.inv = inv;
     g.mask = s->mask;
+    g.np2 = s->np2;
     return 0;
 }
 
@@ -668,7 +831,7 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     R *w = reinterpret_cast<R *>(s->w);
     R *z = reinterpret_cast<R *>(s->z);
     const unsigned spmv_grid = (unsigned)((n * SUB + 255) / 256);
-    auto prec = [&](const R *x, R *y) { k_bj_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, x, y); };
+    auto prec = [&](const R *x, R *y) { apply_prec<R>(s, inv, x, y, st); };
     double bnorm = 0;
     {
         k_norm2<R><<<nb, RB, 0, st>>>(n, rhs, s->partial);

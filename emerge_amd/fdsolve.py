@@ -179,11 +179,34 @@ class FrequencyProblem:
     def to_scipy(self, vals):
         return self.pat.to_scipy(vals)
 
+    def gradient(self):
+        """Device CSR of the P2 gradient on the free DOFs (built once per problem)."""
+        if getattr(self, "_gradient", None) is None:
+            from types import SimpleNamespace
+            from .gradient import gradient_matrix
+            torch = self.torch
+            G, nodes = gradient_matrix(self.mesh, self.mask.cpu().numpy())
+            Gt = G.T.tocsr()
+            Gt.sort_indices()
+            d = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a).astype(dt)).to(self.dev)
+            self._gradient = SimpleNamespace(np2=G.shape[1], nodes=nodes, host=G,
+                                             rowptr=d(G.indptr, np.int32), cols=d(G.indices, np.int32),
+                                             vals=d(G.data, np.float64), t_rowptr=d(Gt.indptr, np.int32),
+                                             t_cols=d(Gt.indices, np.int32), t_vals=d(Gt.data, np.float64))
+        return self._gradient
+
 
 class GPUSolver:
-    """Restarted GMRES on the GPU (emg_solver_* in the C ABI)."""
+    """Restarted GMRES on the GPU (emg_solver_* in the C ABI).
 
-    def __init__(self, problem: FrequencyProblem, restart: int = 150, precision: int = 64):
+    preconditioner: "hybrid" (2x2 block Jacobi plus the gradient correction
+    G diag(G^T K G)^{-1} G^T, emerge_amd.gradient) or "jacobi" (block Jacobi only).
+    """
+
+    def __init__(self, problem: FrequencyProblem, restart: int = 150, precision: int = 64,
+                 preconditioner: str = "hybrid"):
+        if preconditioner not in ("hybrid", "jacobi"):
+            raise ValueError(f"preconditioner must be 'hybrid' or 'jacobi', not {preconditioner!r}")
         self.L = _lib.load()
         self.p = problem
         h = ctypes.c_void_p()
@@ -192,6 +215,13 @@ class GPUSolver:
                    "emg_solver_create")
         self.h = h
         self.precision = precision
+        self.preconditioner = preconditioner
+        if preconditioner == "hybrid":
+            G = problem.gradient()
+            _lib.check(self.L.emg_solver_set_gradient(self.h, G.np2, _lib.ptr(G.rowptr), _lib.ptr(G.cols),
+                                                      _lib.ptr(G.vals), _lib.ptr(G.t_rowptr), _lib.ptr(G.t_cols),
+                                                      _lib.ptr(G.t_vals), _lib.stream_ptr()),
+                       "emg_solver_set_gradient")
 
     def __del__(self):
         h = getattr(self, "h", None)

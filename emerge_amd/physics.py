@@ -31,15 +31,17 @@ class GPUSolveSettings:
            (GMRES for every frequency and port, warm-started from the previous
            frequency)
     max_basis: cap on the number of stored basis vectors for sweep="basis"
+    preconditioner: "hybrid" (2x2 block Jacobi + gradient-space correction) or "jacobi"
     """
 
     def __init__(self, precision: int = 32, tol: float = 1e-8, restart: int = 150, maxiter: int = 60000,
                  max_refine: int = 0, warm_start: bool = True, store_fields: bool = True, sweep: str = "basis",
-                 max_basis: int = 128):
+                 max_basis: int = 128, preconditioner: str = "hybrid"):
         if sweep not in ("basis", "plain"):
             raise ValueError(f"sweep must be 'basis' or 'plain', not {sweep!r}")
         self.sweep = sweep
         self.max_basis = max_basis
+        self.preconditioner = preconditioner
         self.precision = precision
         self.tol = tol
         self.restart = restart
@@ -141,7 +143,7 @@ class Electrodynamics3D:
         st = self.solveroutine
         prob = self.problem()
         sampler = PortSampler(prob)
-        solver = GPUSolver(prob, restart=st.restart, precision=st.precision)
+        solver = GPUSolver(prob, restart=st.restart, precision=st.precision, preconditioner=st.preconditioner)
         ports = prob.ports
         nums = [p.port_number for p in ports]
         data = EMSimData()

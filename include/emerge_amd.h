@@ -97,6 +97,16 @@ int emg_solver_create(int64_t n, int64_t nnz, const int32_t *d_rowptr, const int
 /* K values (nnz complex128, caller-owned, must outlive the solves) and the
  * free-DOF mask (n uint8, 1 = solve, 0 = PEC; NULL = all free). */
 int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const uint8_t *d_mask, void *stream);
+/* Gradient correction of the smoother (no counterpart in the reference, whose
+ * direct solvers need none): G (n x np2, CSR, at most 8 entries per row) maps
+ * second-order Lagrange nodes to the Nedelec DOFs (the null space of the
+ * curl-curl matrix), G^T (np2 x n) is its transpose; values float64,
+ * caller-owned, must outlive the solver; columns touching masked DOFs must be
+ * left out.  From the next emg_solver_set_matrix on, the preconditioner is
+ * z = BJ(r) + G diag(G^T K G)^{-1} G^T r.  np2 = 0 switches it off.          */
+int emg_solver_set_gradient(emg_solver *s, int64_t np2, const int32_t *d_g_rowptr, const int32_t *d_g_cols,
+                            const double *d_g_vals, const int32_t *d_gt_rowptr, const int32_t *d_gt_cols,
+                            const double *d_gt_vals, void *stream);
 /* d_x: initial guess in, solution out (n complex128). Synchronises `stream`. */
 int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, double tol, int maxiter, int max_refine,
                      emg_solve_info *info, void *stream);
