@@ -356,21 +356,30 @@ __global__ void __launch_bounds__(RB) k_dsinv(int64_t np2, const double *__restr
     if (inv32) { inv32[2 * p] = (float)r; inv32[2 * p + 1] = (float)im; }
 }
 
-// t[p] = dsinv[p] * sum over row p of G^T of g * r
+// t[p] = dsinv[p] * sum over row p of G^T of g * r  (16-lane group per row:
+// vertex rows of G^T hold tens of entries)
 template <typename R>
-__global__ void __launch_bounds__(RB) k_gt_apply(int64_t np2, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                 const R *__restrict__ gv, const R *__restrict__ dsinv,
-                                                 const R *__restrict__ r, R *__restrict__ t) {
-    const int64_t p = blockIdx.x * (int64_t)RB + threadIdx.x;
+__global__ void __launch_bounds__(256) k_gt_apply(int64_t np2, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                  const R *__restrict__ gv, const R *__restrict__ dsinv,
+                                                  const R *__restrict__ r, R *__restrict__ t) {
+    const int64_t p = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / SUB;
+    const int lane = threadIdx.x % SUB;
     if (p >= np2) return;
     double ar = 0.0, ai = 0.0;
-    for (int32_t e = rp[p]; e < rp[p + 1]; e++) {
+    for (int32_t e = rp[p] + lane; e < rp[p + 1]; e += SUB) {
         const cx<R> v = ld(r, ci[e]);
         ar += (double)gv[e] * v.re;
         ai += (double)gv[e] * v.im;
     }
-    const cx<R> d = ld(dsinv, p);
-    st(t, p, cx<R>((R)(d.re * ar - d.im * ai), (R)(d.re * ai + d.im * ar)));
+#pragma unroll
+    for (int off = SUB / 2; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off, SUB);
+        ai += __shfl_xor(ai, off, SUB);
+    }
+    if (lane == 0) {
+        const cx<R> d = ld(dsinv, p);
+        st(t, p, cx<R>((R)(d.re * ar - d.im * ai), (R)(d.re * ai + d.im * ar)));
+    }
 }
 
 // z = BJ(r) + G t, both modes of entity e per thread
@@ -542,7 +551,7 @@ static void apply_prec(emg_solver *s, const R *inv, const R *x, R *y, hipStream_
         const R *gtv = sizeof(R) == 8 ? reinterpret_cast<const R *>(s->gt_vals) : reinterpret_cast<const R *>(s->gt_vals32);
         const R *dsi = sizeof(R) == 8 ? reinterpret_cast<const R *>(s->dsinv64) : reinterpret_cast<const R *>(s->dsinv32);
         R *t = reinterpret_cast<R *>(s->tg);
-        k_gt_apply<R><<<emg::grid_for(s->np2, RB), RB, 0, st>>>(s->np2, s->gt_rowptr, s->gt_cols, gtv, dsi, x, t);
+        k_gt_apply<R><<<(unsigned)((s->np2 * SUB + 255) / 256), 256, 0, st>>>(s->np2, s->gt_rowptr, s->gt_cols, gtv, dsi, x, t);
         k_bj_grad_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, x, s->g_rowptr, s->g_cols, gv, t, y);
     } else {
         k_bj_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, x, y);

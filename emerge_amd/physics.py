@@ -32,16 +32,21 @@ class GPUSolveSettings:
            frequency)
     max_basis: cap on the number of stored basis vectors for sweep="basis"
     preconditioner: "hybrid" (2x2 block Jacobi + gradient-space correction) or "jacobi"
+    basis_tol: GMRES solves that enlarge the basis go to tol * basis_tol
+    verbose: one progress line per block of the basis sweep on stderr
     """
 
     def __init__(self, precision: int = 32, tol: float = 1e-8, restart: int = 150, maxiter: int = 60000,
                  max_refine: int = 0, warm_start: bool = True, store_fields: bool = True, sweep: str = "basis",
-                 max_basis: int = 128, preconditioner: str = "hybrid"):
+                 max_basis: int = 128, preconditioner: str = "hybrid", basis_tol: float = 1e-1,
+                 verbose: bool = False):
         if sweep not in ("basis", "plain"):
             raise ValueError(f"sweep must be 'basis' or 'plain', not {sweep!r}")
         self.sweep = sweep
         self.max_basis = max_basis
         self.preconditioner = preconditioner
+        self.basis_tol = basis_tol
+        self.verbose = verbose
         self.precision = precision
         self.tol = tol
         self.restart = restart
@@ -154,7 +159,7 @@ class Electrodynamics3D:
         if st.sweep == "basis":
             from .sweep import BasisSweep
             bs = BasisSweep(prob, solver, sampler, tol=st.tol, max_basis=st.max_basis, maxiter=st.maxiter,
-                            max_refine=st.max_refine)
+                            max_refine=st.max_refine, basis_tol=st.basis_tol, verbose=st.verbose)
             res = bs.run(list(self.frequencies), label)
             for fi, freq in enumerate(self.frequencies):
                 k0 = 2 * np.pi * freq / C0
@@ -175,6 +180,7 @@ class Electrodynamics3D:
             torch.cuda.synchronize()
             data.info["sweep_seconds"] = time.perf_counter() - t0
             data.info["basis_size"] = bs.m
+            data.info["trial_log"] = bs.log
             self.data = data
             return data
         for freq in self.frequencies:

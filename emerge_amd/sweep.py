@@ -50,7 +50,7 @@ def _c128_to_f64(a: np.ndarray) -> np.ndarray:
 
 class BasisSweep:
     def __init__(self, problem, solver, sampler, tol: float = 1e-8, max_basis: int = 128, maxiter: int = 60000,
-                 max_refine: int = 0, block: int = MAX_BLOCK, basis_tol: float = 1e-2):
+                 max_refine: int = 0, block: int = MAX_BLOCK, basis_tol: float = 1e-1, verbose: bool = False):
         import torch
         self.torch = torch
         self.p = problem
@@ -58,6 +58,7 @@ class BasisSweep:
         self.sampler = sampler
         self.tol = tol
         self.basis_tol = basis_tol
+        self.verbose = verbose
         self.maxiter = maxiter
         self.max_refine = max_refine
         self.L = _lib.load()
@@ -124,6 +125,7 @@ class BasisSweep:
         self.q_pd = q_host[:, self.pdofs]                      # (nports, npd)
         self.pdofs_d = torch.from_numpy(self.pdofs.astype(np.int64)).to(dev)
         self.Vpd = np.zeros((max_basis, npd), dtype=np.complex128)
+        self.log: list[tuple] = []          # (basis size, [(freq index, port index, trial residual)])
         # K(f) V = U C(f) with U = [E v_i, B v_i, P_t v_i]_i; U = Q R (Q orthonormal, in HBM)
         self.ncol = 2 + len(self.terms)
         qmax = max_basis * self.ncol
@@ -305,6 +307,7 @@ class BasisSweep:
                     break
                 Y = [self.trial(freqs[fi], pi) for fi, pi in todo]
                 res = self.residuals([(freqs[fi], pi) for fi, pi in todo], Y)
+                self.log.append((self.m, [(fi, pi, float(r)) for (fi, pi), r in zip(todo, res)]))
                 still = []
                 for (fi, pi), y, r in zip(todo, Y, res):
                     if r <= self.tol:
@@ -320,6 +323,11 @@ class BasisSweep:
                 out[fi][pi] = (y, SolveRecord(self.ports[pi].port_number, info.iterations, info.refinements,
                                               info.residual, bool(info.converged), label))
                 todo = [t for t, _ in still if t != (fi, pi)]
+            if self.verbose:
+                import sys
+                nfull = sum(1 for row in out for e in row if e is not None and e[1].iterations > 0)
+                print(f"sweep: block {b0 // self.block + 1}/{(len(rest) + self.block - 1) // self.block} done, "
+                      f"basis {self.m}, GMRES solves {nfull}", file=sys.stderr, flush=True)
         return out
 
     def fields_of(self, y: np.ndarray) -> np.ndarray:

@@ -86,6 +86,11 @@ class Pardiso:
         if err.value != 0:
             raise RuntimeError(f"PARDISO phase {phase} error {err.value}")
 
+    @property
+    def memory_kb(self) -> int:
+        """Peak memory of analysis + factorisation in KB (iparm(15), iparm(16)+iparm(17))."""
+        return int(max(self.iparm[14], self.iparm[15] + self.iparm[16]))
+
     def solve(self, b: np.ndarray) -> np.ndarray:
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.zeros_like(b)
