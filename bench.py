@@ -54,7 +54,7 @@ def cpu_baseline(mesh, eps, sample_tets: int):
 def sweep_workload(name: str):
     """(mesh, port factory, frequencies, description) of a sweep workload."""
     from emerge_amd import meshgen
-    from emerge_amd.bc import CoordinateSystem, LumpedPort, RectangularWaveguide
+    from emerge_amd.bc import CoordinateSystem, RectangularWaveguide
     if name == "wr90":
         a, b, L = 22.86e-3, 10.16e-3, 50e-3
         mesh = meshgen.waveguide_wr90()
@@ -66,14 +66,21 @@ def sweep_workload(name: str):
                                           "2 rectangular waveguide ports, air")
     if name == "posts":
         mesh = meshgen.posts_box()
-        W, H, X = meshgen.PORT_WIDTH, meshgen.PORT_HEIGHT, 40e-3
-        cs1 = CoordinateSystem((0, 1, 0), (0, 0, 1), (1, 0, 0), (0.0, 10e-3, H / 2))
-        cs2 = CoordinateSystem((0, -1, 0), (0, 0, 1), (-1, 0, 0), (X, 10e-3, H / 2))
-        ports = lambda: [LumpedPort([meshgen.PORT1_TAG], 1, width=W, height=H, direction=(0, 0, 1), cs=cs1, Z0=50),
-                         LumpedPort([meshgen.PORT2_TAG], 2, width=W, height=H, direction=(0, 0, 1), cs=cs2, Z0=50)]
-        return mesh, ports, (2e9, 4e9), ("posts box 40x20x10 mm, 4 PEC posts r 1.5 mm h 8 mm, 80x40x16 Kuhn hexes "
-                                         f"carved to {mesh.n_tets:,} tets, 2 lumped ports 2x3.75 mm (50 ohm), air")
+        return mesh, lambda: sweep_ports(mesh), (2e9, 4e9), (
+            "posts box 40x20x10 mm, 4 PEC posts r 1.5 mm h 8 mm, 80x40x16 Kuhn hexes "
+            f"carved to {mesh.n_tets:,} tets, 2 lumped ports 2x3.75 mm (50 ohm) on the x faces, air")
     raise ValueError(name)
+
+
+def sweep_ports(mesh):
+    """The two lumped ports of the posts box (strips from the floor on x = 0 and x = 40 mm)."""
+    from emerge_amd import meshgen
+    from emerge_amd.bc import CoordinateSystem, LumpedPort
+    W, H, X = meshgen.PORT_WIDTH, meshgen.PORT_HEIGHT, 40e-3
+    cs1 = CoordinateSystem((0, 1, 0), (0, 0, 1), (1, 0, 0), (0.0, 10e-3, H / 2))
+    cs2 = CoordinateSystem((0, -1, 0), (0, 0, 1), (-1, 0, 0), (X, 10e-3, H / 2))
+    return [LumpedPort([meshgen.PORT1_TAG], 1, width=W, height=H, direction=(0, 0, 1), cs=cs1, Z0=50),
+            LumpedPort([meshgen.PORT2_TAG], 2, width=W, height=H, direction=(0, 0, 1), cs=cs2, Z0=50)]
 
 
 def bench_sweep(args, dev):
@@ -121,27 +128,65 @@ def bench_sweep(args, dev):
         out["max_abs_dS_basis_vs_plain"] = float(np.abs(runs["basis_mixed"][:k] - runs["plain_fp64"]).max())
         out["speedup_basis_vs_plain"] = out["basis_mixed"]["freq_pts_per_s"] / out["plain_fp64"]["freq_pts_per_s"]
     if args.sweep_cpu:
-        out["cpu_baseline"] = cpu_sweep_baseline(mesh, phys, freqs[len(freqs) // 2], desc)
+        out["cpu_baseline"] = cpu_sweep_baseline(args.sweep, mesh, phys, freqs[len(freqs) // 2], desc)
         out["speedup_basis_vs_cpu"] = out["basis_mixed"]["freq_pts_per_s"] / out["cpu_baseline"]["value"]
     return out
 
 
-def cpu_sweep_baseline(mesh, phys, freq, desc):
-    """The reference CPU path for one frequency of the same problem: oracle
-    (C/OpenMP) assembly, MKL PARDISO on the real block system (the reference's
-    x86 default solver), both port solves and S-parameters -- host cores."""
+def cpu_sweep_baseline(name, mesh, phys, freq, desc):
+    """The reference CPU path for one frequency: oracle (C/OpenMP) assembly,
+    MKL PARDISO on the real block system (the reference's x86 default solver),
+    both port solves and S-parameters -- host cores.
+
+    WR-90: the same problem.  Posts box: its PARDISO factorisation needs
+    ~120 GB and minutes, so the same geometry is timed at 40x20x8 and 60x30x12
+    hexes and the time per frequency is extrapolated to 80x40x16 along the
+    power law through the two points (in the number of real unknowns)."""
     from oracle import fd
+    from emerge_amd.fdsolve import pec_dofs
     cores = min(len(os.sched_getaffinity(0)), 16)
-    er, ur = phys.material_tensors()
-    t0 = time.perf_counter()
-    prob = fd.OracleProblem(mesh, er, ur, phys.boundary_conditions, solver="pardiso", threads=cores)
-    prob.frequency_domain([freq])
-    prob.release()
-    dt = time.perf_counter() - t0
+
+    def one(m, ph):
+        from emerge_amd.bc import LumpedPort
+        from emerge_amd.sparam import lumped_port_line
+        for p in ph.ports():
+            if isinstance(p, LumpedPort) and p.vintline is None:
+                lumped_port_line(m, p)
+        er, ur = ph.material_tensors()
+        t0 = time.perf_counter()
+        prob = fd.OracleProblem(m, er, ur, ph.boundary_conditions, solver="pardiso", threads=cores)
+        prob.frequency_domain([freq])
+        prob.release()
+        return time.perf_counter() - t0
+
+    def n_real(m, ph):
+        return 2 * (2 * m.n_entities - pec_dofs(m, ph.boundary_conditions).shape[0])
+
+    if name != "posts":
+        dt = one(mesh, phys)
+        return {"value": 1.0 / dt, "unit": "freq-pts/s", "cores": cores, "kind": "port",
+                "sample": f"one frequency ({freq / 1e9:.3f} GHz) of the same problem ({desc}): oracle C/OpenMP "
+                          f"assembly, MKL PARDISO factorisation of the real 2N block system ({cores} threads), "
+                          f"2 port solves, S-parameters: {dt:.1f} s"}
+    from emerge_amd import meshgen
+    from emerge_amd.physics import Electrodynamics3D
+    pts = []
+    for hx in ((40, 20, 8), (60, 30, 12)):
+        m = meshgen.posts_box(*hx)
+        ph = Electrodynamics3D(m)
+        ph.assign(*sweep_ports(m))
+        pts.append((n_real(m, ph), one(m, ph), hx))
+        print(json.dumps({"cpu_sweep_point": {"hexes": hx, "real_unknowns": pts[-1][0], "seconds": pts[-1][1]}}),
+              file=sys.stderr, flush=True)
+    n_full = n_real(mesh, phys)
+    (n1, t1, h1), (n2, t2, h2) = pts
+    expo = np.log(t2 / t1) / np.log(n2 / n1)
+    dt = t2 * (n_full / n2) ** expo
     return {"value": 1.0 / dt, "unit": "freq-pts/s", "cores": cores, "kind": "port",
-            "sample": f"one frequency ({freq / 1e9:.3f} GHz) of the same problem ({desc}): oracle C/OpenMP assembly, "
-                      f"MKL PARDISO factorisation of the real 2N block system ({cores} threads), "
-                      f"2 port solves, S-parameters: {dt:.1f} s"}
+            "sample": f"PARDISO path ({cores} threads) timed on the same geometry at {h1} hexes "
+                      f"({n1:,} real unknowns, {t1:.1f} s/frequency) and {h2} ({n2:,}, {t2:.1f} s); "
+                      f"extrapolated to {n_full:,} real unknowns with exponent {expo:.2f}: {dt:.0f} s/frequency",
+            "extrapolated": True}
 
 
 def main():
@@ -153,10 +198,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-face-mass", action="store_true")
     ap.add_argument("--two-phase", action="store_true", help="time the per-tet-block fallback kernels")
-    ap.add_argument("--sweep", default="wr90", choices=["none", "wr90", "posts"],
+    ap.add_argument("--sweep", default="posts", choices=["none", "wr90", "posts"],
                     help="frequency-sweep workload reported beside the assembly figure")
-    ap.add_argument("--sweep-freqs", type=int, default=41)
-    ap.add_argument("--plain-sample", type=int, default=5, help="frequencies solved by plain FP64 GMRES")
+    ap.add_argument("--sweep-freqs", type=int, default=201)
+    ap.add_argument("--plain-sample", type=int, default=2, help="frequencies solved by plain FP64 GMRES")
     ap.add_argument("--sweep-tol", type=float, default=1e-8)
     ap.add_argument("--sweep-cpu", action="store_true", help="time the host-core reference on one sweep frequency")
     ap.add_argument("--copy-calibration", action="store_true",

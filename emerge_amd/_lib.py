@@ -40,6 +40,17 @@ SIGNATURES: dict[str, tuple] = {
     "emg_mcomb": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P]),
     "emg_robin_blocks": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P, P]),
     "emg_interp_points": (ctypes.c_int, [I64, P, P, P, P, P, I64, ctypes.c_int, P, P, P]),
+    "emg_solver_set_weights": (ctypes.c_int, [P, P, P]),
+    "emg_solver_smoother_parts": (ctypes.c_int, [P, P, P, P, P]),
+    "emg_csr_real_apply": (ctypes.c_int, [I64, P, P, P, P, P, ctypes.c_int, P]),
+    "emg_gnn_gemm": (ctypes.c_int, [I64, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_float, P, P]),
+    "emg_gnn_gemm_nt": (ctypes.c_int, [I64, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_float, P, P]),
+    "emg_gnn_gemm_tn": (ctypes.c_int, [I64, ctypes.c_int, ctypes.c_int, P, P, P, P, P]),
+    "emg_gnn_agg": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P, P, P]),
+    "emg_gnn_relu": (ctypes.c_int, [I64, P, P, P, P, P, P]),
+    "emg_gnn_readout": (ctypes.c_int, [I64, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P]),
+    "emg_gnn_weights": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P]),
+    "emg_gnn_weight_grad": (ctypes.c_int, [I64, ctypes.c_int, I64, I64, P, P, P, P, P]),
     "emg_sweep_residual": (ctypes.c_int, [I64, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P]),
     "emg_port_apply": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P, P, P, P]),
     "emg_basis_combine": (ctypes.c_int, [I64, ctypes.c_int, P, ctypes.c_int, P, P, P]),
@@ -81,8 +92,15 @@ def check(rc: int, what: str) -> None:
 
 
 def ptr(t) -> ctypes.c_void_p:
-    """Device (or host) pointer of a torch tensor / numpy array."""
+    """Device (or host) pointer of a torch tensor / numpy array.
+
+    Lazily conjugated or negated torch views and non-contiguous tensors are
+    refused: their memory does not hold the values the view shows in order."""
     if hasattr(t, "data_ptr"):
+        if (hasattr(t, "is_conj") and t.is_conj()) or (hasattr(t, "is_neg") and t.is_neg()):
+            raise EmgError("pointer of a lazily conjugated/negated tensor view; call resolve_conj() first")
+        if not t.is_contiguous():
+            raise EmgError("pointer of a non-contiguous tensor; call contiguous() first")
         return ctypes.c_void_p(t.data_ptr())
     return t.ctypes.data_as(ctypes.c_void_p)
 

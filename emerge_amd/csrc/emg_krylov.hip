@@ -524,6 +524,9 @@ struct emg_solver {
     float *g_vals32 = nullptr, *gt_vals32 = nullptr;      // owned copies for complex64 steps
     double *dS = nullptr, *dsinv64 = nullptr, *tg = nullptr;
     float *dsinv32 = nullptr;
+    // optional learned weights of the smoother (caller-owned complex128):
+    // w_e per entity scales its 2x2 block inverse, w_p per P2 node its diagonal
+    const double *w_e = nullptr, *w_p = nullptr;
     struct graphs {
         std::vector<hipGraphExec_t> exec;
         const void *vals = nullptr, *inv = nullptr, *mask = nullptr;
@@ -619,6 +622,31 @@ extern "C" int emg_solver_create(int64_t n, int64_t nnz, const int32_t *d_rowptr
 
 extern "C" void emg_solver_destroy(emg_solver *s) { free_solver(s); }
 
+// inv blocks of entity e *= w_e[e] (complex64 copy too); dsinv[p] *= w_p[p]
+__global__ void __launch_bounds__(RB) k_scale_tables(int64_t n_ent, const double *__restrict__ we, double *__restrict__ inv64,
+                                                     float *__restrict__ inv32, int64_t np2, const double *__restrict__ wp,
+                                                     double *__restrict__ ds64, float *__restrict__ ds32) {
+    const int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x;
+    if (we != nullptr && i < n_ent) {
+        const double wr = we[2 * i], wi = we[2 * i + 1];
+        for (int q = 0; q < 4; q++) {
+            double *a = inv64 + 2 * (4 * i + q);
+            const double ar = a[0], ai = a[1];
+            a[0] = wr * ar - wi * ai;
+            a[1] = wr * ai + wi * ar;
+            if (inv32) { inv32[2 * (4 * i + q)] = (float)a[0]; inv32[2 * (4 * i + q) + 1] = (float)a[1]; }
+        }
+    }
+    if (wp != nullptr && i < np2) {
+        const double wr = wp[2 * i], wi = wp[2 * i + 1];
+        double *a = ds64 + 2 * i;
+        const double ar = a[0], ai = a[1];
+        a[0] = wr * ar - wi * ai;
+        a[1] = wr * ai + wi * ar;
+        if (ds32) { ds32[2 * i] = (float)a[0]; ds32[2 * i + 1] = (float)a[1]; }
+    }
+}
+
 extern "C" int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const uint8_t *d_mask, void *stream) {
     EMG_CHECK(s && d_vals, "emg_solver_set_matrix: null argument");
     hipStream_t st = (hipStream_t)stream;
@@ -632,6 +660,31 @@ extern "C" int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const 
                                                          s->g_cols, s->g_vals, s->dS);
         k_dsinv<<<emg::grid_for(s->np2, RB), RB, 0, st>>>(s->np2, s->dS, s->dsinv64, s->dsinv32);
     }
+    if (s->w_e != nullptr || (s->w_p != nullptr && s->np2 > 0)) {
+        const int64_t nmax = std::max<int64_t>(s->n_ent, s->np2);
+        k_scale_tables<<<emg::grid_for(nmax, RB), RB, 0, st>>>(s->n_ent, s->w_e, s->inv64, s->inv32, s->np2,
+                                                               s->np2 > 0 ? s->w_p : nullptr, s->dsinv64, s->dsinv32);
+    }
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int emg_solver_set_weights(emg_solver *s, const double *d_we, const double *d_wp) {
+    EMG_CHECK(s != nullptr, "emg_solver_set_weights: null solver");
+    s->w_e = d_we;
+    s->w_p = d_wp;
+    return 0;
+}
+
+// The two parts of the smoother for a vector r (complex128): u = BJ(r) (n) and,
+// with the gradient correction set, t = diag(G^T K G)^{-1} G^T r (np2).
+extern "C" int emg_solver_smoother_parts(emg_solver *s, const double *d_r, double *d_u, double *d_t, void *stream) {
+    EMG_CHECK(s && d_r && d_u, "emg_solver_smoother_parts: null argument");
+    hipStream_t st = (hipStream_t)stream;
+    k_bj_apply<double><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, s->inv64, d_r, d_u);
+    if (s->np2 > 0 && d_t != nullptr)
+        k_gt_apply<double><<<(unsigned)((s->np2 * SUB + 255) / 256), 256, 0, st>>>(s->np2, s->gt_rowptr, s->gt_cols,
+                                                                                  s->gt_vals, s->dsinv64, d_r, d_t);
     EMG_LAUNCH_CHECK();
     return 0;
 }
@@ -1049,6 +1102,38 @@ extern "C" int emg_maxpy(int64_t n, int nv, const double *d_V, const double *d_h
     const int nb = nblk_for(n);
     k_maxpy_norm<double><<<nb, RB, 0, st>>>(n, nv, d_V, n, d_h, d_w, d_work, nullptr);
     k_scale_by_norm<double><<<1, RB, 0, st>>>(0, d_work, nb, d_w, d_w, d_norm);   // n = 0: norm only
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+// y (+)= A x for a real CSR matrix A (float64 values) and complex128 x, y.
+__global__ void __launch_bounds__(256) k_csr_real(int64_t nrows, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                  const double *__restrict__ av, const double *__restrict__ x,
+                                                  double *__restrict__ y, int accumulate) {
+    const int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / SUB;
+    const int lane = threadIdx.x % SUB;
+    if (r >= nrows) return;
+    double ar = 0.0, ai = 0.0;
+    for (int32_t e = rp[r] + lane; e < rp[r + 1]; e += SUB) {
+        ar += av[e] * x[2 * ci[e]];
+        ai += av[e] * x[2 * ci[e] + 1];
+    }
+    for (int off = SUB / 2; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off, SUB);
+        ai += __shfl_xor(ai, off, SUB);
+    }
+    if (lane == 0) {
+        if (accumulate) { y[2 * r] += ar; y[2 * r + 1] += ai; }
+        else { y[2 * r] = ar; y[2 * r + 1] = ai; }
+    }
+}
+
+extern "C" int emg_csr_real_apply(int64_t nrows, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_vals,
+                                  const double *d_x, double *d_y, int accumulate, void *stream) {
+    EMG_CHECK(nrows >= 0 && d_rowptr && d_cols && d_vals && d_x && d_y, "emg_csr_real_apply: null argument");
+    if (nrows == 0) return 0;
+    k_csr_real<<<(unsigned)((nrows * SUB + 255) / 256), 256, 0, (hipStream_t)stream>>>(nrows, d_rowptr, d_cols, d_vals,
+                                                                                     d_x, d_y, accumulate);
     EMG_LAUNCH_CHECK();
     return 0;
 }

@@ -232,15 +232,38 @@ class GPUSolver:
                 pass
             self.h = None
 
-    def set_matrix(self, K):
+    learned = None          # LearnedSmoother whose weights scale the smoother at each frequency
+
+    def set_weights(self, we, wp):
+        """Learned smoother weights (complex128 device tensors or None) for the next set_matrix."""
+        self._w = (we, wp)
+        _lib.check(self.L.emg_solver_set_weights(self.h, _lib.ptr(we) if we is not None else None,
+                                                 _lib.ptr(wp) if wp is not None else None), "emg_solver_set_weights")
+
+    def set_matrix(self, K, freq: float | None = None):
+        """K(f) values; with a learned smoother attached and `freq` given, its weights for f are applied."""
         self._K = K
+        if self.learned is not None and freq is not None:
+            self.set_weights(*self.learned.weights(freq))
         _lib.check(self.L.emg_solver_set_matrix(self.h, _lib.ptr(K), _lib.ptr(self.p.mask), _lib.stream_ptr()),
                    "emg_solver_set_matrix")
 
     def solve(self, b, x0=None, tol: float = 1e-8, maxiter: int = 20000, max_refine: int = 0):
+        """GMRES from x0; with learned smoother weights active, a solve that does not converge is
+        continued from its last iterate with the plain smoother (info.reserved = 1 marks it)."""
         torch = self.p.torch
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         info = _lib.SolveInfo()
         _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, maxiter, max_refine,
                                            ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
+        w = getattr(self, "_w", (None, None))
+        if not info.converged and (w[0] is not None or w[1] is not None):
+            used = info.iterations
+            self.set_weights(None, None)
+            _lib.check(self.L.emg_solver_set_matrix(self.h, _lib.ptr(self._K), _lib.ptr(self.p.mask),
+                                                    _lib.stream_ptr()), "emg_solver_set_matrix")
+            _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, maxiter, max_refine,
+                                               ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
+            info.iterations += used
+            info.reserved = 1
         return x, info

@@ -34,12 +34,13 @@ class GPUSolveSettings:
     preconditioner: "hybrid" (2x2 block Jacobi + gradient-space correction) or "jacobi"
     basis_tol: GMRES solves that enlarge the basis go to tol * basis_tol
     verbose: one progress line per block of the basis sweep on stderr
+    learned: a trained emerge_amd.learned.LearnedSmoother (None = plain hybrid smoother)
     """
 
     def __init__(self, precision: int = 32, tol: float = 1e-8, restart: int = 150, maxiter: int = 60000,
                  max_refine: int = 0, warm_start: bool = True, store_fields: bool = True, sweep: str = "basis",
                  max_basis: int = 128, preconditioner: str = "hybrid", basis_tol: float = 1e-1,
-                 verbose: bool = False):
+                 verbose: bool = False, learned=None):
         if sweep not in ("basis", "plain"):
             raise ValueError(f"sweep must be 'basis' or 'plain', not {sweep!r}")
         self.sweep = sweep
@@ -47,6 +48,7 @@ class GPUSolveSettings:
         self.preconditioner = preconditioner
         self.basis_tol = basis_tol
         self.verbose = verbose
+        self.learned = learned
         self.precision = precision
         self.tol = tol
         self.restart = restart
@@ -149,6 +151,10 @@ class Electrodynamics3D:
         prob = self.problem()
         sampler = PortSampler(prob)
         solver = GPUSolver(prob, restart=st.restart, precision=st.precision, preconditioner=st.preconditioner)
+        if st.learned is not None:
+            if st.preconditioner != "hybrid":
+                raise SimulationError("a learned smoother needs preconditioner='hybrid'")
+            solver.learned = st.learned
         ports = prob.ports
         nums = [p.port_number for p in ports]
         data = EMSimData()
@@ -192,7 +198,7 @@ class Electrodynamics3D:
             for p in ports:
                 ds.add_port_properties(p.port_number, p.mode_number, k0, p.get_beta(k0), p.Z0, p.power)
             K = prob.system(freq)
-            solver.set_matrix(K)
+            solver.set_matrix(K, freq)
             X = []
             for ap in ports:
                 b = prob.rhs(ap, freq)

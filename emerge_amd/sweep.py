@@ -274,7 +274,7 @@ class BasisSweep:
     def full_solve(self, freq: float, port_index: int, y0: np.ndarray | None):
         p = self.p
         if getattr(self, "_matrix_freq", None) != freq:
-            self.solver.set_matrix(p.system(freq))
+            self.solver.set_matrix(p.system(freq), freq)
             self._matrix_freq = freq
         b = p.rhs(self.ports[port_index], freq)
         x0 = None

@@ -107,6 +107,15 @@ int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const uint8_t *d_
 int emg_solver_set_gradient(emg_solver *s, int64_t np2, const int32_t *d_g_rowptr, const int32_t *d_g_cols,
                             const double *d_g_vals, const int32_t *d_gt_rowptr, const int32_t *d_gt_cols,
                             const double *d_gt_vals, void *stream);
+/* Learned smoother weights (no counterpart in the reference): complex128 w_e
+ * per entity (n/2) multiplies that entity's 2x2 block inverse, w_p per P2
+ * node (np2) its entry of diag(G^T K G)^{-1}; either may be NULL.  Pointers are
+ * kept (caller-owned) and applied by every following emg_solver_set_matrix.  */
+int emg_solver_set_weights(emg_solver *s, const double *d_we, const double *d_wp);
+/* The smoother's two parts for r (complex128): d_u = BJ(r) (n) and, when the
+ * gradient correction is set and d_t is not NULL, d_t = diag(G^T K G)^{-1}
+ * G^T r (np2), with the tables of the last emg_solver_set_matrix.           */
+int emg_solver_smoother_parts(emg_solver *s, const double *d_r, double *d_u, double *d_t, void *stream);
 /* d_x: initial guess in, solution out (n complex128). Synchronises `stream`. */
 int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, double tol, int maxiter, int max_refine,
                      emg_solve_info *info, void *stream);
@@ -130,6 +139,9 @@ int emg_mdotc(int64_t n, int nv, const double *d_V, const double *d_w, double *d
 int emg_maxpy(int64_t n, int nv, const double *d_V, const double *d_h, double *d_w, double *d_norm, double *d_work,
               void *stream);
 int emg_mcomb(int64_t n, int nv, const double *d_V, const double *d_y, double *d_x, void *stream);
+/* y (+)= A x, A real CSR (float64 values), x and y complex128. */
+int emg_csr_real_apply(int64_t nrows, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_vals,
+                       const double *d_x, double *d_y, int accumulate, void *stream);
 
 /* ------------------------------------------------------------------ boundary terms
  * Robin (port / absorbing boundary) triangle terms, replacing
@@ -181,6 +193,36 @@ int emg_port_apply(int64_t nprow, int nvec, const int32_t *d_ptr, const int32_t 
 /* X[row, j] = sum_i V[i, row] Y[j, i]: d_V (m, n) c128 basis, d_Y (nvec, m) c128,
  * nvec * m <= 2048.                                                            */
 int emg_basis_combine(int64_t n, int m, const double *d_V, int nvec, const double *d_Y, double *d_X, void *stream);
+
+/* ------------------------------------------------------------------ learned smoother
+ * Kernels of the graph network that predicts the smoother weights from mesh
+ * features (no counterpart in the reference).  Activations float32 row-major,
+ * layer weights float16 (k x m row-major), float32 accumulation.
+ *   gemm:      C[n x m] = A[n x k] W + bias + beta C          (k <= 128)
+ *   gemm_nt:   out[n x k] = D[n x m] W^T + beta out           (m <= 128)
+ *   gemm_tn:   dW[k x m] += A^T D, dbias[m] += column sums of D (either NULL-able bias)
+ *   agg:       Y[v] = rs[v] sum_{u in N(v)} cs[u] X[u]  over a vertex CSR (rs, cs NULL = 1)
+ *   relu:      forward H = relu(P) + Hprev (d_dH NULL), backward dP = dH . (P > 0)
+ *   readout:   forward O[e, c] = bias[c] + sum_q Hp[v_q(e), off + q C + c] (K <= 3 slots),
+ *              backward (d_dO given): dHp and dbias accumulate (atomics)
+ *   weights:   forward (d_dw NULL) w = 1 + O[., 0] + j O[., 1] (mode 0) or exp(O[., 0]) (mode 1),
+ *              complex128; backward dO from d(Re w, Im w) (float32 pairs)
+ *   weight_grad: dO[i] += 2 (Re, Im) sum_s scale_s conj(a_s[i]) b_s[i] (+ DOF i + stride)   */
+int emg_gnn_gemm(int64_t n, int k, int m, const float *d_A, const void *d_W, const float *d_bias, float beta,
+                 float *d_C, void *stream);
+int emg_gnn_gemm_nt(int64_t n, int m, int k, const float *d_D, const void *d_W, float beta, float *d_out, void *stream);
+int emg_gnn_gemm_tn(int64_t n, int k, int m, const float *d_A, const float *d_D, float *d_dW, float *d_dbias,
+                    void *stream);
+int emg_gnn_agg(int64_t nv, int C, const int32_t *d_ptr, const int32_t *d_idx, const float *d_rs, const float *d_cs,
+                const float *d_X, float *d_Y, void *stream);
+int emg_gnn_relu(int64_t n, const float *d_P, const float *d_Hprev, float *d_H, const float *d_dH, float *d_dP,
+                 void *stream);
+int emg_gnn_readout(int64_t ne, int K, int C, const int32_t *d_ev, const float *d_Hp, int ldh, int off,
+                    const float *d_bias, float *d_O, const float *d_dO, float *d_dHp, float *d_dbias, void *stream);
+int emg_gnn_weights(int64_t n, int mode, const float *d_O, double *d_w, const float *d_dw, float *d_dO,
+                    void *stream);
+int emg_gnn_weight_grad(int64_t n, int ns, int64_t ld, int64_t stride, const double *d_a, const double *d_b,
+                        const double *d_scale, float *d_dO, void *stream);
 
 #ifdef __cplusplus
 }
