@@ -91,6 +91,9 @@ def bench_sweep(args, dev):
       certificates for the rest -- all `sweep_freqs` points;
     * plain: complex128 GMRES for every (frequency, port), warm-started from the
       previous frequency -- the first `plain_sample` points, rate extrapolated;
+    * learned: the basis sweep with the learned smoother weights, the network
+      trained first on plain-solver residuals at the band's ends and middle
+      (training time reported separately);
     * cpu_baseline: the host-core reference path on one frequency."""
     import torch
     from emerge_amd.physics import Electrodynamics3D
@@ -99,6 +102,9 @@ def bench_sweep(args, dev):
     out = {"workload": desc, "tets": mesh.n_tets, "dofs": 2 * mesh.n_entities, "freqs": len(freqs),
            "band_hz": [f0, f1], "tol": args.sweep_tol}
     runs = {}
+    if args.sweep_kernel_only:
+        out["residual_kernel"] = residual_kernel_roofline(mesh, ports, freqs, args.sweep_tol)
+        return out
     for key, mode, prec, fl in (("basis_mixed", "basis", 32, freqs), ("plain_fp64", "plain", 64, freqs[: args.plain_sample])):
         if not fl:
             continue
@@ -106,7 +112,7 @@ def bench_sweep(args, dev):
         phys.assign(*ports())
         phys.set_frequency(fl)
         st = phys.solveroutine
-        st.precision, st.tol, st.store_fields, st.sweep = prec, args.sweep_tol, False, mode
+        st.precision, st.tol, st.store_fields, st.sweep, st.verbose = prec, args.sweep_tol, False, mode, True
         phys.problem()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -123,14 +129,132 @@ def bench_sweep(args, dev):
         if mode == "basis":
             out[key]["basis_size"] = int(data.info.get("basis_size", 0))
         print(json.dumps({key: out[key]}), file=sys.stderr, flush=True)
+    if args.learned:
+        from emerge_amd.fdsolve import GPUSolver
+        from emerge_amd.learned import LearnedSmoother
+        phys = Electrodynamics3D(mesh)
+        phys.assign(*ports())
+        prob = phys.problem()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        net = LearnedSmoother(prob)
+        s64 = GPUSolver(prob, precision=64)
+        train_f = [f0, 0.5 * (f0 + f1), f1]
+        log = lambda m: print(m, file=sys.stderr, flush=True)
+        samples = net.collect(s64, train_f)
+        log(f"learned: collected {sum(r.shape[0] for _, r, _ in samples)} residual samples at {len(train_f)} frequencies")
+        prep = net.prepare(s64, samples)
+        del samples
+        hist = net.train(prep, steps=args.learned_steps, lr=3e-3, log=log)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        del prep, s64
+        torch.cuda.empty_cache()
+        phys.set_frequency(freqs)
+        st = phys.solveroutine
+        st.precision, st.tol, st.store_fields, st.sweep, st.learned = 32, args.sweep_tol, False, "basis", net
+        st.verbose, st.maxiter = True, 20000
+        t0 = time.perf_counter()
+        data = phys.frequency_domain()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        recs = [r for ds in data.datasets for r in ds.solves]
+        runs["learned"] = data.S_array()
+        out["learned_basis_mixed"] = {
+            "freq_pts_per_s": len(freqs) / dt, "seconds": dt, "freqs": len(freqs),
+            "gmres_solves": int(sum(r.iterations > 0 for r in recs)),
+            "iterations_total": int(sum(r.iterations for r in recs)),
+            "max_fp64_residual": float(max(r.residual for r in recs)),
+            "all_converged": bool(all(r.converged and r.residual <= args.sweep_tol for r in recs)),
+            "basis_size": int(data.info.get("basis_size", 0)),
+            "learned_check": data.info.get("learned"),
+            "training": {"seconds": t_train, "frequencies_hz": train_f, "steps": args.learned_steps,
+                         "loss_first": hist[0], "loss_last": hist[-1]},
+            "max_abs_dS_vs_basis": float(np.abs(runs["learned"] - runs["basis_mixed"]).max())}
+        print(json.dumps({"learned_basis_mixed": out["learned_basis_mixed"]}), file=sys.stderr, flush=True)
+    out["residual_kernel"] = residual_kernel_roofline(mesh, ports, freqs, args.sweep_tol)
     if "plain_fp64" in runs:
         k = runs["plain_fp64"].shape[0]
         out["max_abs_dS_basis_vs_plain"] = float(np.abs(runs["basis_mixed"][:k] - runs["plain_fp64"]).max())
         out["speedup_basis_vs_plain"] = out["basis_mixed"]["freq_pts_per_s"] / out["plain_fp64"]["freq_pts_per_s"]
+        if "learned_basis_mixed" in out:
+            out["speedup_learned_vs_plain"] = (out["learned_basis_mixed"]["freq_pts_per_s"]
+                                               / out["plain_fp64"]["freq_pts_per_s"])
     if args.sweep_cpu:
-        out["cpu_baseline"] = cpu_sweep_baseline(args.sweep, mesh, phys, freqs[len(freqs) // 2], desc)
+        out["cpu_baseline"] = cpu_sweep_child(args.sweep, freqs[len(freqs) // 2])
         out["speedup_basis_vs_cpu"] = out["basis_mixed"]["freq_pts_per_s"] / out["cpu_baseline"]["value"]
+        if "learned_basis_mixed" in out:
+            out["speedup_learned_vs_cpu"] = out["learned_basis_mixed"]["freq_pts_per_s"] / out["cpu_baseline"]["value"]
     return out
+
+
+def residual_kernel_roofline(mesh, ports, freqs, tol, nvec: int = 16, reps: int = 20):
+    """The sweep's batched FP64 certificate kernel (k_sweep_residual, one affine SpMM
+    pass over E, B and the CSR indices for 16 (frequency, port) pairs) timed with HIP
+    events on the stream it runs on, against HBM peak.
+
+    Algorithmic bytes per launch: nnz (E 16 + B 16 + column 4) + rows (row pointer 4 +
+    mask 1 + row map 4) + the trial block read once (16 nvec per row) + the port
+    sources read by each pair (16 nvec per row) -- the residual itself is not written."""
+    import torch
+    from emerge_amd import _lib
+    from emerge_amd.fdsolve import GPUSolver
+    from emerge_amd.physics import Electrodynamics3D
+    from emerge_amd.sparam import PortSampler
+    from emerge_amd.sweep import BasisSweep
+    phys = Electrodynamics3D(mesh)
+    phys.assign(*ports())
+    prob = phys.problem()
+    bs = BasisSweep(prob, GPUSolver(prob, precision=64, preconditioner="jacobi"), PortSampler(prob), max_basis=8)
+    rng = np.random.default_rng(0)
+    mask = prob.mask.cpu().numpy()
+    for _ in range(4):
+        v = (rng.standard_normal(prob.N) + 1j * rng.standard_normal(prob.N)) * mask
+        bs.add(torch.from_numpy(v).to(prob.dev))
+    pairs = [(float(freqs[(7 * j) % len(freqs)]), j % len(bs.ports)) for j in range(nvec)]
+    Y = [rng.standard_normal(bs.m) + 1j * rng.standard_normal(bs.m) for _ in pairs]
+    bs.residuals(pairs, Y)                                   # warm-up (also builds the block)
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    # time the kernel alone: the block X and coefficients are prepared by the first call's code path
+    L = _lib.load()
+    n, nv = prob.N, nvec
+    X = torch.randn(n * nv, dtype=torch.complex128, device=prob.dev)
+    X = X.view(n, nv).mul_(prob.mask[:, None]).reshape(-1).contiguous()
+    sd = torch.tensor([(2 * np.pi * f / 299792458.0) ** 2 for f, _ in pairs], dtype=torch.float64, device=prob.dev)
+    cd = torch.ones(nv, dtype=torch.complex128, device=prob.dev)
+    qd = torch.tensor([pi for _, pi in pairs], dtype=torch.int32, device=prob.dev)
+    zp = torch.zeros(max(bs.nprow, 1) * nv, dtype=torch.complex128, device=prob.dev)
+    for i in range(reps):
+        ev[i][0].record(stream)
+        _lib.check(L.emg_sweep_residual(n, _lib.ptr(prob.pat.rowptr), _lib.ptr(prob.pat.cols), _lib.ptr(prob.E),
+                                        _lib.ptr(prob.B), _lib.ptr(prob.mask), nv, _lib.ptr(sd), _lib.ptr(X),
+                                        _lib.ptr(bs.qsrc), _lib.ptr(qd), _lib.ptr(cd), _lib.ptr(bs.prow_of_row),
+                                        _lib.ptr(zp), None, _lib.ptr(bs.nrm), _lib.ptr(bs.work), _lib.stream_ptr()),
+                   "emg_sweep_residual")
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    nnz = prob.nnz
+    alg = nnz * 36 + n * (4 + 1 + 4) + n * nv * 16 * 2
+    ach = alg / (ms * 1e-3) / 1e9
+    return {"kernel": "k_sweep_residual<16>", "pairs_per_launch": nv, "kernel_ms": ms, "alg_bytes": alg,
+            "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "pairs_per_s": nv / (ms * 1e-3)}
+
+
+def cpu_sweep_child(name: str, freq: float) -> dict:
+    """Run cpu_sweep_baseline in a child process that never touches the GPU (MKL's
+    threads and allocator then run as they do in a plain host program)."""
+    import subprocess
+    env = dict(os.environ)
+    cores = str(min(len(os.sched_getaffinity(0)), 16))
+    env.update(OMP_NUM_THREADS=cores, MKL_NUM_THREADS=cores)
+    cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sweep-child", name, "--cpu-sweep-freq", repr(freq)],
+                        env=env, stdout=subprocess.PIPE, text=True)
+    if cp.returncode != 0:
+        raise RuntimeError(f"host-core sweep baseline failed ({cp.returncode})")
+    return json.loads(cp.stdout.strip().splitlines()[-1])
 
 
 def cpu_sweep_baseline(name, mesh, phys, freq, desc):
@@ -204,9 +328,23 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=2, help="frequencies solved by plain FP64 GMRES")
     ap.add_argument("--sweep-tol", type=float, default=1e-8)
     ap.add_argument("--sweep-cpu", action="store_true", help="time the host-core reference on one sweep frequency")
+    ap.add_argument("--no-learned", dest="learned", action="store_false",
+                    help="skip training / using the learned smoother in the sweep")
+    ap.add_argument("--learned-steps", type=int, default=60)
+    ap.add_argument("--sweep-kernel-only", action="store_true",
+                    help="measure only the batched residual kernel of the sweep (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device-to-device copy (reference for PMC byte counters)")
+    ap.add_argument("--cpu-sweep-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-sweep-freq", type=float, default=3e9, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_sweep_child:
+        from emerge_amd.physics import Electrodynamics3D
+        mesh, ports, _, desc = sweep_workload(args.cpu_sweep_child)
+        phys = Electrodynamics3D(mesh)
+        phys.assign(*ports())
+        print(json.dumps(cpu_sweep_baseline(args.cpu_sweep_child, mesh, phys, args.cpu_sweep_freq, desc)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the frequency-sharded "
               "multi-GPU design is described in the docs but not run here", file=sys.stderr)

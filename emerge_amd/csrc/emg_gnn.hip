@@ -151,12 +151,16 @@ __global__ void __launch_bounds__(GB) k_readout_bwd(int64_t ne, int K, int C, co
     if (dbias) atomicAdd(&dbias[c], g);
 }
 
-// Weights from the readout, complex128: mode 0 w = 1 + O0 + j O1, mode 1 w = exp(O0) (real, positive).
+// Weights from the readout, complex128: mode 0 w = 1 + O0 + j O1, mode 1 w = exp(O0) (real, positive),
+// mode 2 w = exp(0.5 tanh(O0)) (real, in [0.61, 1.65]).
 __global__ void __launch_bounds__(GB) k_weights(int64_t n, int mode, const float *__restrict__ O, double *__restrict__ w) {
     const int64_t t = blockIdx.x * (int64_t)GB + threadIdx.x;
     if (t >= n) return;
     if (mode == 1) {
         w[2 * t] = exp((double)O[2 * t]);
+        w[2 * t + 1] = 0.0;
+    } else if (mode == 2) {
+        w[2 * t] = exp(0.5 * tanh((double)O[2 * t]));
         w[2 * t + 1] = 0.0;
     } else {
         w[2 * t] = 1.0 + (double)O[2 * t];
@@ -171,6 +175,10 @@ __global__ void __launch_bounds__(GB) k_weights_bwd(int64_t n, int mode, const f
     if (t >= n) return;
     if (mode == 1) {
         dO[2 * t] = dw[2 * t] * expf(O[2 * t]);
+        dO[2 * t + 1] = 0.0f;
+    } else if (mode == 2) {
+        const float th = tanhf(O[2 * t]);
+        dO[2 * t] = dw[2 * t] * expf(0.5f * th) * 0.5f * (1.0f - th * th);
         dO[2 * t + 1] = 0.0f;
     } else {
         dO[2 * t] = dw[2 * t];
@@ -274,7 +282,7 @@ extern "C" int emg_gnn_readout(int64_t ne, int K, int C, const int32_t *d_ev, co
 
 extern "C" int emg_gnn_weights(int64_t n, int mode, const float *d_O, double *d_w, const float *d_dw, float *d_dO,
                                void *stream) {
-    EMG_CHECK(n >= 0 && (mode == 0 || mode == 1) && d_O, "emg_gnn_weights: bad argument");
+    EMG_CHECK(n >= 0 && mode >= 0 && mode <= 2 && d_O, "emg_gnn_weights: bad argument");
     if (n == 0) return 0;
     if (d_dw) {
         EMG_CHECK(d_dO != nullptr, "emg_gnn_weights: backward needs d_dO");

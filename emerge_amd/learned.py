@@ -39,8 +39,9 @@ PROJ = 16
 
 
 class LearnedSmoother:
-    def __init__(self, problem, hidden: int = 64, layers: int = 4, seed: int = 0, mode: int = 1):
-        """mode 1: real positive weights exp(o) (default); mode 0: complex weights 1 + o."""
+    def __init__(self, problem, hidden: int = 64, layers: int = 4, seed: int = 0, mode: int = 2):
+        """mode 2: real weights exp(0.5 tanh(o)) in [0.61, 1.65] (default); mode 1: exp(o);
+        mode 0: complex weights 1 + o."""
         import torch
         self.mode = mode
         self.torch = torch
@@ -223,7 +224,7 @@ class LearnedSmoother:
         return self.readouts_to_weights(Ov, Oe, Of)
 
     # ------------------------------------------------------------ training
-    def collect(self, solver, freqs, iters=(0, 2, 8, 32), tol: float = 1e-10, probes: int = 2, seed: int = 0):
+    def collect(self, solver, freqs, iters=(0, 2, 8, 32), tol: float = 1e-9, probes: int = 2, seed: int = 0):
         """Residual samples of the plain smoother's GMRES at each frequency: r_k = b - K x_k
         after k iterations from zero, for every port, with the pairs' solution errors x* - x_k;
         plus `probes` masked white-noise vectors (their errors K^{-1} r are not needed by the

@@ -187,6 +187,7 @@ class Electrodynamics3D:
             data.info["sweep_seconds"] = time.perf_counter() - t0
             data.info["basis_size"] = bs.m
             data.info["trial_log"] = bs.log
+            data.info["learned"] = bs.learned_info
             self.data = data
             return data
         for freq in self.frequencies:

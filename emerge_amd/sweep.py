@@ -126,6 +126,7 @@ class BasisSweep:
         self.pdofs_d = torch.from_numpy(self.pdofs.astype(np.int64)).to(dev)
         self.Vpd = np.zeros((max_basis, npd), dtype=np.complex128)
         self.log: list[tuple] = []          # (basis size, [(freq index, port index, trial residual)])
+        self.learned_info = None            # learned-vs-plain check of the first GMRES solve
         # K(f) V = U C(f) with U = [E v_i, B v_i, P_t v_i]_i; U = Q R (Q orthonormal, in HBM)
         self.ncol = 2 + len(self.terms)
         qmax = max_basis * self.ncol
@@ -283,6 +284,27 @@ class BasisSweep:
             self._comb(self.V, self.m, y0, x0)
         x, info = self.solver.solve(b, x0=x0, tol=self.tol * self.basis_tol, maxiter=self.maxiter,
                                     max_refine=self.max_refine)
+        if self.solver.learned is not None and self.learned_info is None:
+            # first GMRES of the sweep with learned weights: solve again with the plain smoother and
+            # keep the network only if it needed fewer iterations (the plain smoother is the fallback)
+            self.solver.set_weights(None, None)
+            self.solver.set_matrix(p.K)
+            x2, info2 = self.solver.solve(b, x0=x0, tol=self.tol * self.basis_tol, maxiter=self.maxiter,
+                                          max_refine=self.max_refine)
+            use = bool(info.converged) and info.iterations < info2.iterations
+            self.learned_info = {"learned_iterations": int(info.iterations), "plain_iterations": int(info2.iterations),
+                                 "used": use}
+            if use:
+                self.solver.set_matrix(p.K, freq)
+            else:
+                self.solver.learned = None
+                x, info2.iterations = x2, info2.iterations + info.iterations
+                info = info2
+        if self.verbose:
+            import sys
+            print(f"sweep: GMRES f={freq:.6g} port {self.ports[port_index].port_number}: {info.iterations} iterations, "
+                  f"residual {info.residual:.2e}{' (plain fallback)' if info.reserved else ''}", file=sys.stderr,
+                  flush=True)
         return x, info
 
     # ------------------------------------------------------------ driver

@@ -205,7 +205,8 @@ int emg_basis_combine(int64_t n, int m, const double *d_V, int nvec, const doubl
  *   relu:      forward H = relu(P) + Hprev (d_dH NULL), backward dP = dH . (P > 0)
  *   readout:   forward O[e, c] = bias[c] + sum_q Hp[v_q(e), off + q C + c] (K <= 3 slots),
  *              backward (d_dO given): dHp and dbias accumulate (atomics)
- *   weights:   forward (d_dw NULL) w = 1 + O[., 0] + j O[., 1] (mode 0) or exp(O[., 0]) (mode 1),
+ *   weights:   forward (d_dw NULL) w = 1 + O[., 0] + j O[., 1] (mode 0), exp(O[., 0]) (mode 1) or
+ *              exp(0.5 tanh(O[., 0])) (mode 2),
  *              complex128; backward dO from d(Re w, Im w) (float32 pairs)
  *   weight_grad: dO[i] += 2 (Re, Im) sum_s scale_s conj(a_s[i]) b_s[i] (+ DOF i + stride)   */
 int emg_gnn_gemm(int64_t n, int k, int m, const float *d_A, const void *d_W, const float *d_bias, float beta,

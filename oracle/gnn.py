@@ -13,7 +13,8 @@ Network (weights rounded to float16 exactly as the device keeps them):
   O_v = Hp[v, 0:2] + c_v
   O_e = Hp[a, 2:6] + Hp[b, 6:10] + c_e          (edge a < b)
   O_f = Hp[a, 10:12] + Hp[b, 12:14] + Hp[c, 14:16] + c_f   (face a < b < c)
-Smoother weights: w = exp(O[:, 0]) (mode 1) or 1 + O[:, 0] + j O[:, 1] (mode 0); entity weight w_e from O_e[:, 0:2] (edges) and
+Smoother weights: w = exp(0.5 tanh(O[:, 0])) (mode 2, default), exp(O[:, 0]) (mode 1) or
+1 + O[:, 0] + j O[:, 1] (mode 0); entity weight w_e from O_e[:, 0:2] (edges) and
 O_f (faces); P2-node weight w_p from O_v (vertices) and O_e[:, 2:4] (edge midpoints).
 Loss over residual samples r_s:  J = sum_s ||r_s - K z_s||^2 / ||r_s||^2,
 z_s = w_e . BJ(r_s) + G (w_p . t_s),  t_s = diag(G^T K G)^{-1} G^T r_s.
@@ -86,19 +87,24 @@ class OracleNet:
         return g
 
 
-def entity_node_weights(Ov, Oe, Of, nodes, nv, mode: int = 1):
+def entity_node_weights(Ov, Oe, Of, nodes, nv, mode: int = 2):
     """(w_e per entity, w_p per kept P2 node) from the readouts: mode 1 exp(o_0), mode 0 1 + o_0 + j o_1."""
     Oent = np.concatenate([Oe[:, 0:2], Of])
     Onode = np.concatenate([Ov, Oe[:, 2:4]])[nodes]
     if mode == 1:
         return np.exp(Oent[:, 0]) + 0j, np.exp(Onode[:, 0]) + 0j
+    if mode == 2:
+        return np.exp(0.5 * np.tanh(Oent[:, 0])) + 0j, np.exp(0.5 * np.tanh(Onode[:, 0])) + 0j
     return 1 + Oent[:, 0] + 1j * Oent[:, 1], 1 + Onode[:, 0] + 1j * Onode[:, 1]
 
 
-def weights_backward(O, dw, mode: int = 1):
+def weights_backward(O, dw, mode: int = 2):
     """d(readout) from d(Re w, Im w)."""
     if mode == 1:
         return np.stack([dw[:, 0] * np.exp(O[:, 0]), np.zeros(O.shape[0])], 1)
+    if mode == 2:
+        th = np.tanh(O[:, 0])
+        return np.stack([dw[:, 0] * np.exp(0.5 * th) * 0.5 * (1 - th * th), np.zeros(O.shape[0])], 1)
     return dw.copy()
 
 

@@ -99,10 +99,14 @@ def test_exponential_weight_backward():
     f = lambda o: np.sum(c * np.exp(o[:, 0]) ** 2)
     dw = np.stack([2 * c * np.exp(O[:, 0]), np.zeros(5)], 1)          # d f / d Re(w)
     g = weights_backward(O, dw, mode=1)
+    f2 = lambda o: np.sum(c * np.exp(0.5 * np.tanh(o[:, 0])) ** 2)
+    th = np.tanh(O[:, 0])
+    g2 = weights_backward(O, np.stack([2 * c * np.exp(0.5 * th), np.zeros(5)], 1), mode=2)
     h = 1e-6
     for i in range(5):
         Op, Om = O.copy(), O.copy()
         Op[i, 0] += h
         Om[i, 0] -= h
         assert abs((f(Op) - f(Om)) / (2 * h) - g[i, 0]) < 1e-6 * max(1, abs(g[i, 0]))
+        assert abs((f2(Op) - f2(Om)) / (2 * h) - g2[i, 0]) < 1e-6 * max(1, abs(g2[i, 0]))
     assert np.all(g[:, 1] == 0)
