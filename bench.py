@@ -250,7 +250,7 @@ def cpu_sweep_child(name: str, freq: float) -> dict:
     env = dict(os.environ)
     cores = str(min(len(os.sched_getaffinity(0)), 16))
     env.update(OMP_NUM_THREADS=cores, MKL_NUM_THREADS=cores)
-    cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sweep-child", name, "--cpu-sweep-freq", repr(freq)],
+    cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sweep-child", name, "--cpu-sweep-freq", str(float(freq))],
                         env=env, stdout=subprocess.PIPE, text=True)
     if cp.returncode != 0:
         raise RuntimeError(f"host-core sweep baseline failed ({cp.returncode})")

@@ -16,8 +16,10 @@ Features are local and frequency-aware (k0 times the mean incident edge
 length), so one trained network serves every frequency of a sweep.  Layer
 weights are float16 on the device with float32 accumulation; float32 master
 copies are updated by Adam.  Forward and backward run in this library's own
-kernels (emg_gnn_*); the loss is the preconditioned residual
-    J = sum_s ||r_s - K z_s||^2 / ||r_s||^2
+kernels (emg_gnn_*); the loss is the preconditioned residual after the best
+global rescaling (GMRES is blind to the smoother's overall scale, so the loss is
+too):
+    J = sum_s min_alpha ||r_s - alpha K z_s||^2 / ||r_s||^2
 over residuals r_s that the plain solver produces at a few GMRES iteration
 counts (each paired with the converged solution's error x* - x_k), so training
 targets the components that plain GMRES has not yet removed.  The weights
@@ -288,16 +290,23 @@ class LearnedSmoother:
         Gv = torch.empty((ns, self.p.N), dtype=torch.complex128, device=self.dev)
         Gtg = torch.empty((ns, G.np2), dtype=torch.complex128, device=self.dev)
         J = 0.0
+        ones = torch.ones(ns, dtype=torch.float64, device=self.dev)
         for s in range(ns):
             wt = (wp * T[s]).contiguous()
             z = W * U[s]
             _lib.check(L.emg_csr_real_apply(self.p.N, P(G.rowptr), P(G.cols), P(G.vals), P(wt), P(z), 1, st),
                        "emg_csr_real_apply")
-            res = R[s] - self._apply_K(K, z)
-            J += float(scale[s]) * float((res.abs() ** 2).sum())
-            Gv[s] = -self._apply_K(K, res, conj=True)          # -K^H res (K complex symmetric)
+            v = self._apply_K(K, z)
+            a = complex(torch.vdot(v, R[s]))                    # v^H r
+            nv = float((v.abs() ** 2).sum())
+            rr = 1.0 / float(scale[s])
+            res = R[s] - (a / nv) * v
+            J += float((res.abs() ** 2).sum()) / rr
+            # dJ/d conj(z) = -(conj(a) / (nv rr)) K^H (r - alpha v)   (K complex symmetric)
+            Gv[s] = (-(a.conjugate() / (nv * rr))) * self._apply_K(K, res, conj=True)
             _lib.check(L.emg_csr_real_apply(G.np2, P(G.t_rowptr), P(G.t_cols), P(G.t_vals), P(Gv[s]), P(Gtg[s]), 0, st),
                        "emg_csr_real_apply")
+        scale = ones
         dent = torch.zeros((n_ent, 2), dtype=torch.float32, device=self.dev)
         dnode = torch.zeros((G.np2, 2), dtype=torch.float32, device=self.dev)
         _lib.check(L.emg_gnn_weight_grad(n_ent, ns, self.p.N, n_ent, P(U), P(Gv), P(scale), P(dent), st),

@@ -16,7 +16,8 @@ Network (weights rounded to float16 exactly as the device keeps them):
 Smoother weights: w = exp(0.5 tanh(O[:, 0])) (mode 2, default), exp(O[:, 0]) (mode 1) or
 1 + O[:, 0] + j O[:, 1] (mode 0); entity weight w_e from O_e[:, 0:2] (edges) and
 O_f (faces); P2-node weight w_p from O_v (vertices) and O_e[:, 2:4] (edge midpoints).
-Loss over residual samples r_s:  J = sum_s ||r_s - K z_s||^2 / ||r_s||^2,
+Loss over residual samples r_s (invariant to a global scale of the smoother, as GMRES is):
+  J = sum_s min_alpha ||r_s - alpha K z_s||^2 / ||r_s||^2 = sum_s (1 - |<K z_s, r_s>|^2 / (||K z_s||^2 ||r_s||^2)),
 z_s = w_e . BJ(r_s) + G (w_p . t_s),  t_s = diag(G^T K G)^{-1} G^T r_s.
 """
 from __future__ import annotations
@@ -116,14 +117,19 @@ def smoother_loss(w_e, w_p, R, U, T, K, G):
     gp = np.zeros((w_p.shape[0], 2))
     W = np.concatenate([w_e, w_e])
     for r, u, t in zip(R, U, T):
-        s = 1.0 / np.vdot(r, r).real
+        rr = np.vdot(r, r).real
         z = W * u + G @ (w_p * t)
-        res = r - K @ z
-        J += s * np.vdot(res, res).real
-        gvec = -(K.conj().T @ res)
-        a = np.conj(u) * gvec
-        de = s * (a[:n_ent] + a[n_ent:])
-        dp = s * np.conj(t) * (G.T @ gvec)
+        v = K @ z
+        a = np.vdot(v, r)                     # v^H r
+        nv = np.vdot(v, v).real
+        alpha = a / nv
+        res = r - alpha * v
+        J += np.vdot(res, res).real / rr
+        # dJ/d conj(v) = -(conj(a) / (nv rr)) (r - alpha v)
+        gvec = -(np.conj(a) / (nv * rr)) * (K.conj().T @ res)
+        a2 = np.conj(u) * gvec
+        de = a2[:n_ent] + a2[n_ent:]
+        dp = np.conj(t) * (G.T @ gvec)
         ge += 2 * np.stack([de.real, de.imag], 1)
         gp += 2 * np.stack([dp.real, dp.imag], 1)
     return J, ge, gp

@@ -66,6 +66,8 @@ def test_smoother_loss_gradient_matches_finite_differences():
     we = 1 + 0.1 * (rng.standard_normal(n_ent) + 1j * rng.standard_normal(n_ent))
     wp = 1 + 0.1 * (rng.standard_normal(np2) + 1j * rng.standard_normal(np2))
     J, ge, gp = smoother_loss(we, wp, R, U, T, K, G)
+    # invariant to a global scale of the smoother
+    assert abs(smoother_loss(3.0 * we, 3.0 * wp, R, U, T, K, G)[0] - J) < 1e-12 * J
     h = 1e-7
     for i in (0, 7, 29):
         for c, d in ((0, 1.0), (1, 1j)):
