@@ -41,13 +41,13 @@ __device__ __forceinline__ void stc(double *p, int64_t i, c2 v) {
 }
 
 constexpr int RES_BLOCK = 256;
-constexpr int ROW_LANES = 16;        // lanes that share one CSR row
+constexpr int ROW_LANES = 64;        // one wavefront per CSR row
 constexpr int MAX_VEC = 16;
 constexpr int RES_GRID_MAX = 4096;   // persistent grid: partial sums per block
 
-// One 16-lane group per row; lane l handles vector j = l % NV over the row's
-// entries l / NV, l / NV + 16 / NV, ...  E, B and the column index are loaded
-// once per entry and shared by the NV vectors.
+// One wavefront per row; lane l handles vector j = l % NV over the row's
+// entries l / NV, l / NV + 64 / NV, ...  E, B and the column index of an entry
+// are loaded once per group of NV lanes and shared by the NV vectors.
 template <int NV>
 __global__ void __launch_bounds__(RES_BLOCK)
     k_sweep_residual(int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
