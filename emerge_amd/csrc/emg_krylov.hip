@@ -77,6 +77,8 @@ __device__ __forceinline__ void st(R *p, int64_t i, cx<R> v) {
 }
 
 static constexpr int SUB = 16;  // lanes per CSR row
+// blocks of 256 threads for k_spmv over n rows, a multiple of 8 (one share per XCD)
+static inline unsigned spmv_blocks(int64_t n) { return (unsigned)(((n * SUB + 255) / 256 + 7) / 8 * 8); }
 
 // y = mask.(A (mask.x)) + (1-mask).x   (mask == nullptr: plain y = A x)
 // If `b` is given, y = b - (that).
@@ -84,8 +86,12 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_spmv(int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
                                               const R *__restrict__ vals, const uint8_t *__restrict__ mask,
                                               const R *__restrict__ x, const R *__restrict__ b, R *__restrict__ y) {
-    const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t row = gt / SUB;
+    // XCD-aware rows: launched with a multiple of 8 blocks, block b takes the
+    // b / 8-th block of XCD (b % 8)'s contiguous share of the rows, so the
+    // columns a block gathers stay in its XCD's L2.
+    const int64_t nbx = gridDim.x / 8;
+    const int64_t blk = (blockIdx.x % 8) * nbx + blockIdx.x / 8;
+    const int64_t row = (blk * blockDim.x + threadIdx.x) / SUB;
     const int lane = threadIdx.x % SUB;
     if (row >= n) return;
     const bool keep = mask == nullptr || mask[row];
@@ -843,7 +849,7 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
     const int nbm = mdot_blocks(n, nv);
     const dim3 grid(nbm, (nv + DT - 1) / DT);
     apply_prec<R>(s, inv, vj, z, st);
-    k_spmv<R><<<(unsigned)((n * SUB + 255) / 256), 256, 0, st>>>(n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn);
+    k_spmv<R><<<spmv_blocks(n), 256, 0, st>>>(n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn);
     k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, nullptr);
     k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, h1, 0, nullptr);
     k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, h1, vn, s->pn, nullptr);
@@ -892,7 +898,7 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     R *V = reinterpret_cast<R *>(s->V);
     R *w = reinterpret_cast<R *>(s->w);
     R *z = reinterpret_cast<R *>(s->z);
-    const unsigned spmv_grid = (unsigned)((n * SUB + 255) / 256);
+    const unsigned spmv_grid = spmv_blocks(n);
     auto prec = [&](const R *x, R *y) { apply_prec<R>(s, inv, x, y, st); };
     double bnorm = 0;
     {
@@ -979,7 +985,7 @@ extern "C" int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, d
     memset(info, 0, sizeof(*info));
     auto true_resid = [&](double *out_rel, double *bn) -> int {
         // r = b - K x in complex128 (masked)
-        k_spmv<double><<<(unsigned)((n * SUB + 255) / 256), 256, 0, st>>>(n, s->rowptr, s->cols, s->vals, s->mask, d_x, d_b, s->r);
+        k_spmv<double><<<spmv_blocks(n), 256, 0, st>>>(n, s->rowptr, s->cols, s->vals, s->mask, d_x, d_b, s->r);
         k_norm2<double><<<nb, RB, 0, st>>>(n, s->r, s->partial);
         k_norm2<double><<<nb, RB, 0, st>>>(n, d_b, s->partial + 1024);
         std::vector<double> p(2048);
@@ -1073,7 +1079,7 @@ extern "C" int emg_scatter_add(int64_t m, const int32_t *d_pos, const double *d_
 extern "C" int emg_spmv(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_vals,
                         const uint8_t *d_mask, const double *d_x, double *d_y, void *stream) {
     EMG_CHECK(d_rowptr && d_cols && d_vals && d_x && d_y, "emg_spmv: null argument");
-    k_spmv<double><<<(unsigned)((n * SUB + 255) / 256), 256, 0, (hipStream_t)stream>>>(n, d_rowptr, d_cols, d_vals,
+    k_spmv<double><<<spmv_blocks(n), 256, 0, (hipStream_t)stream>>>(n, d_rowptr, d_cols, d_vals,
                                                                                      d_mask, d_x, nullptr, d_y);
     EMG_LAUNCH_CHECK();
     return 0;
