@@ -243,6 +243,34 @@ def residual_kernel_roofline(mesh, ports, freqs, tol, nvec: int = 16, reps: int 
             "pairs_per_s": nv / (ms * 1e-3)}
 
 
+def bench_eigen(args):
+    """Eigenmodes of the PEC cavity with an eps_r = 6 block (221,184 tets): the modes
+    nearest 5 GHz by shift-invert Arnoldi with GPU GMRES inner solves and FP64
+    Rayleigh-Ritz; spurious (gradient) modes filtered."""
+    import torch
+    from emerge_amd import meshgen
+    from emerge_amd.bc import Material
+    from emerge_amd.physics import Electrodynamics3D
+    mesh = meshgen.cavity_block()
+    phys = Electrodynamics3D(mesh)
+    phys.set_material(meshgen.BLOCK_TAG, Material(er=6.0))
+    phys._bc_initialized = True
+    phys.solveroutine.verbose = True
+    phys.problem()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    data = phys.eigenmode(num_sols=args.eigen_modes, target_frequency=5e9)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    its = data.info["inner_iterations"]
+    return {"workload": "PEC cavity 30x20x15 mm, eps_r 6 block 10x10x5 mm 3 mm from a corner, 48x32x24 Kuhn hexes "
+                        f"({mesh.n_tets:,} tets, {2 * mesh.n_entities:,} DOFs)", "modes": len(data),
+            "seconds": dt, "modes_per_s": len(data) / dt, "target_hz": 5e9,
+            "freqs_hz": [float(ds.freq) for ds in data.datasets],
+            "relative_residuals": [float(ds.residual) for ds in data.datasets],
+            "inner_solves": len(its), "inner_iterations_mean": float(np.mean(its)) if its else 0.0}
+
+
 def cpu_sweep_child(name: str, freq: float) -> dict:
     """Run cpu_sweep_baseline in a child process that never touches the GPU (MKL's
     threads and allocator then run as they do in a plain host program)."""
@@ -331,6 +359,8 @@ def main():
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip training / using the learned smoother in the sweep")
     ap.add_argument("--learned-steps", type=int, default=60)
+    ap.add_argument("--eigen", action="store_true", help="also run the cavity eigenmode workload")
+    ap.add_argument("--eigen-modes", type=int, default=10)
     ap.add_argument("--sweep-kernel-only", action="store_true",
                     help="measure only the batched residual kernel of the sweep (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -428,6 +458,9 @@ def main():
         del E, B, pat
         torch.cuda.empty_cache()
         out["sweep"] = bench_sweep(args, dev)
+    if args.eigen:
+        torch.cuda.empty_cache()
+        out["eigen"] = bench_eigen(args)
     print(json.dumps(out), flush=True)
 
 

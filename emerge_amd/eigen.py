@@ -31,7 +31,7 @@ from .bc import C0
 
 class ShiftInvertArnoldi:
     def __init__(self, problem, sigma: float, ncv: int = 40, inner_tol: float = 1e-12, restart: int = 150,
-                 inner_maxiter: int = 200000, seed: int = 0, preconditioner: str = "hybrid"):
+                 inner_maxiter: int = 200000, seed: int = 0, preconditioner: str = "hybrid", verbose: bool = False):
         import torch
         from .fdsolve import GPUSolver
         self.torch = torch
@@ -53,6 +53,7 @@ class ShiftInvertArnoldi:
         self.h = torch.empty(2 * (ncv + 1), dtype=torch.float64, device=dev)
         self.nrm = torch.empty(4, dtype=torch.float64, device=dev)
         self.seed = seed
+        self.verbose = verbose
         self.inner_iters = []
         self.inner_resid = []
 
@@ -110,6 +111,10 @@ class ShiftInvertArnoldi:
             z, info = self.solver.solve(y, tol=self.inner_tol, maxiter=self.inner_maxiter)
             self.inner_iters.append(info.iterations)
             self.inner_resid.append(info.residual)
+            if self.verbose:
+                import sys
+                print(f"arnoldi: step {j + 1}/{m}, inner GMRES {info.iterations} iterations, residual {info.residual:.1e}",
+                      file=sys.stderr, flush=True)
             hn, hcol = self._orth(j + 1, z)
             H[: j + 1, j] = hcol
             H[j + 1, j] = hn

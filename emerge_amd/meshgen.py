@@ -169,3 +169,20 @@ def posts_box(nx=80, ny=40, nz=16, seed=2, size=(40e-3, 20e-3, 10e-3), radius=1.
         mesh.face_tags[tag] = tri[inside]
         mesh.face_tags[side] = tri[~inside]
     return mesh
+
+
+BLOCK_TAG = 2
+
+
+def cavity_block(nx=48, ny=32, nz=24, seed=3, size=(30e-3, 20e-3, 15e-3), block=((3e-3, 13e-3), (3e-3, 13e-3), (3e-3, 8e-3))):
+    """Eigenmode workload: PEC box 30 x 20 x 15 mm with a dielectric block
+    10 x 10 x 5 mm placed 3 mm from one corner; tets with centroid inside the
+    block carry volume tag BLOCK_TAG (the rest 1).  48 x 32 x 24 hexes give
+    221,184 tets."""
+    (x0, x1), (y0, y1), (z0, z1) = block
+
+    def tag(c):
+        inside = (c[0] > x0) & (c[0] < x1) & (c[1] > y0) & (c[1] < y1) & (c[2] > z0) & (c[2] < z1)
+        return np.where(inside, BLOCK_TAG, 1)
+
+    return structured_box(nx, ny, nz, size=size, jitter=0.05, seed=seed, tag_fn=tag)

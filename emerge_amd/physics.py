@@ -234,7 +234,8 @@ class Electrodynamics3D:
             raise SimulationError("eigenmode needs a target frequency")
         sigma = (2 * np.pi * f_t / C0) ** 2                                      # emfreq3d.py:838-839
         prob = self.problem()
-        arn = ShiftInvertArnoldi(prob, sigma, ncv=ncv or max(2 * num_sols + 10, 24), inner_tol=inner_tol)
+        arn = ShiftInvertArnoldi(prob, sigma, ncv=ncv or max(4 * num_sols + 20, 24), inner_tol=inner_tol,
+                                 preconditioner=self.solveroutine.preconditioner, verbose=self.solveroutine.verbose)
         lam, X, resid = arn.run(num_sols, spurious_rel=spurious_rel)
         data = EMSimData()
         for i in range(lam.shape[0]):
