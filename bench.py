@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sweep freq-pts/s at ≤1e-8 FP64 residual, learned vs plain GPU; assembly Mtet/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_assemble_fused launch on the 1,005,000-tet workload, from
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this benchmark
+# (--sweep none --copy-calibration): FETCH 881,966 KiB x 2 + WRITE 8,791,550 KiB x 1,
+# the factors a 2 GiB device-to-device copy in the same runs gives.
+ASSEMBLY_TRAFFIC_BYTES = (881_966 * 2 + 8_791_550) * 1024
 FP64_PEAK_TFLOPS = 78.6
 # FP64 flop convention per tet for the dense contraction of the closed form
 FLOP_PER_TET = 15449
@@ -105,8 +110,9 @@ def bench_sweep(args, dev):
     if args.sweep_kernel_only:
         out["residual_kernel"] = residual_kernel_roofline(mesh, ports, freqs, args.sweep_tol)
         return out
+    runs_wanted = set(args.sweep_runs.split(","))
     for key, mode, prec, fl in (("basis_mixed", "basis", 32, freqs), ("plain_fp64", "plain", 64, freqs[: args.plain_sample])):
-        if not fl:
+        if not fl or mode not in runs_wanted:
             continue
         phys = Electrodynamics3D(mesh)
         phys.assign(*ports())
@@ -129,7 +135,7 @@ def bench_sweep(args, dev):
         if mode == "basis":
             out[key]["basis_size"] = int(data.info.get("basis_size", 0))
         print(json.dumps({key: out[key]}), file=sys.stderr, flush=True)
-    if args.learned:
+    if args.learned and "learned" in runs_wanted:
         from emerge_amd.fdsolve import GPUSolver
         from emerge_amd.learned import LearnedSmoother
         phys = Electrodynamics3D(mesh)
@@ -170,10 +176,11 @@ def bench_sweep(args, dev):
             "learned_check": data.info.get("learned"),
             "training": {"seconds": t_train, "frequencies_hz": train_f, "steps": args.learned_steps,
                          "loss_first": hist[0], "loss_last": hist[-1]},
-            "max_abs_dS_vs_basis": float(np.abs(runs["learned"] - runs["basis_mixed"]).max())}
+            "max_abs_dS_vs_basis": (float(np.abs(runs["learned"] - runs["basis_mixed"]).max())
+                                    if "basis_mixed" in runs else None)}
         print(json.dumps({"learned_basis_mixed": out["learned_basis_mixed"]}), file=sys.stderr, flush=True)
     out["residual_kernel"] = residual_kernel_roofline(mesh, ports, freqs, args.sweep_tol)
-    if "plain_fp64" in runs:
+    if "plain_fp64" in runs and "basis_mixed" in runs:
         k = runs["plain_fp64"].shape[0]
         out["max_abs_dS_basis_vs_plain"] = float(np.abs(runs["basis_mixed"][:k] - runs["plain_fp64"]).max())
         out["speedup_basis_vs_plain"] = out["basis_mixed"]["freq_pts_per_s"] / out["plain_fp64"]["freq_pts_per_s"]
@@ -182,7 +189,8 @@ def bench_sweep(args, dev):
                                                / out["plain_fp64"]["freq_pts_per_s"])
     if args.sweep_cpu:
         out["cpu_baseline"] = cpu_sweep_child(args.sweep, freqs[len(freqs) // 2])
-        out["speedup_basis_vs_cpu"] = out["basis_mixed"]["freq_pts_per_s"] / out["cpu_baseline"]["value"]
+        if "basis_mixed" in out:
+            out["speedup_basis_vs_cpu"] = out["basis_mixed"]["freq_pts_per_s"] / out["cpu_baseline"]["value"]
         if "learned_basis_mixed" in out:
             out["speedup_learned_vs_cpu"] = out["learned_basis_mixed"]["freq_pts_per_s"] / out["cpu_baseline"]["value"]
     return out
@@ -359,6 +367,8 @@ def main():
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip training / using the learned smoother in the sweep")
     ap.add_argument("--learned-steps", type=int, default=60)
+    ap.add_argument("--sweep-runs", default="basis,plain,learned",
+                    help="comma-separated sweep variants to run: basis, plain, learned")
     ap.add_argument("--eigen", action="store_true", help="also run the cavity eigenmode workload")
     ap.add_argument("--eigen-modes", type=int, default=10)
     ap.add_argument("--sweep-kernel-only", action="store_true",
@@ -447,7 +457,8 @@ def main():
                    "symbolic_pattern_s": round(t_sym, 3), "host_mesh_s": round(t_mesh, 3),
                    "face_mass": "exact" if args.exact_face_mass else "reference"},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": ASSEMBLY_TRAFFIC_BYTES if not (args.two_phase or args.exact_face_mass) else None,
                      "kernel": "k_tet_blocks + k_gather_iso" if args.two_phase else "k_assemble_fused", "kernel_ms": kern_ms, "alg_bytes": alg_bytes,
                      "fp64_tflops": flops / (kern_ms * 1e-3) / 1e12,
                      "fp64_frac": flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
