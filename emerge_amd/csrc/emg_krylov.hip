@@ -77,8 +77,9 @@ __device__ __forceinline__ void st(R *p, int64_t i, cx<R> v) {
 }
 
 static constexpr int SUB = 16;  // lanes per CSR row
-// blocks of 256 threads for k_spmv over n rows, a multiple of 8 (one share per XCD)
-static inline unsigned spmv_blocks(int64_t n) { return (unsigned)(((n * SUB + 255) / 256 + 7) / 8 * 8); }
+// blocks of 256 threads for k_spmv over n rows (rows in order: giving each XCD a
+// contiguous share of the rows fetched a third fewer bytes but ran 30 % slower)
+static inline unsigned spmv_blocks(int64_t n) { return (unsigned)((n * SUB + 255) / 256); }
 
 // y = mask.(A (mask.x)) + (1-mask).x   (mask == nullptr: plain y = A x)
 // If `b` is given, y = b - (that).
@@ -86,12 +87,7 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_spmv(int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
                                               const R *__restrict__ vals, const uint8_t *__restrict__ mask,
                                               const R *__restrict__ x, const R *__restrict__ b, R *__restrict__ y) {
-    // XCD-aware rows: launched with a multiple of 8 blocks, block b takes the
-    // b / 8-th block of XCD (b % 8)'s contiguous share of the rows, so the
-    // columns a block gathers stay in its XCD's L2.
-    const int64_t nbx = gridDim.x / 8;
-    const int64_t blk = (blockIdx.x % 8) * nbx + blockIdx.x / 8;
-    const int64_t row = (blk * blockDim.x + threadIdx.x) / SUB;
+    const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / SUB;
     const int lane = threadIdx.x % SUB;
     if (row >= n) return;
     const bool keep = mask == nullptr || mask[row];
