@@ -77,33 +77,32 @@ __device__ __forceinline__ void st(R *p, int64_t i, cx<R> v) {
 }
 
 static constexpr int SUB = 16;  // lanes per CSR row
-// blocks of 256 threads for k_spmv over n rows (rows in order: giving each XCD a
-// contiguous share of the rows fetched a third fewer bytes but ran 30 % slower)
-static inline unsigned spmv_blocks(int64_t n) { return (unsigned)((n * SUB + 255) / 256); }
+// (SpMV rows stay in dispatch order: giving each XCD a contiguous share of the
+// rows fetched a third fewer bytes but ran 30 % slower.)
 
 // y = mask.(A (mask.x)) + (1-mask).x   (mask == nullptr: plain y = A x)
-// If `b` is given, y = b - (that).
-template <typename R>
+// If `b` is given, y = b - (that).  L lanes share one CSR row.
+template <typename R, int L = SUB>
 __global__ void __launch_bounds__(256) k_spmv(int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
                                               const R *__restrict__ vals, const uint8_t *__restrict__ mask,
                                               const R *__restrict__ x, const R *__restrict__ b, R *__restrict__ y) {
-    const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / SUB;
-    const int lane = threadIdx.x % SUB;
+    const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / L;
+    const int lane = threadIdx.x % L;
     if (row >= n) return;
     const bool keep = mask == nullptr || mask[row];
     cx<R> acc;
     if (keep) {
         const int32_t p0 = rowptr[row], p1 = rowptr[row + 1];
-        for (int32_t p = p0 + lane; p < p1; p += SUB) {
+        for (int32_t p = p0 + lane; p < p1; p += L) {
             const int32_t c = cols[p];
             if (mask != nullptr && !mask[c]) continue;
             acc = cadd(acc, cmul(ld(vals, p), ld(x, c)));
         }
     }
 #pragma unroll
-    for (int off = SUB / 2; off > 0; off >>= 1) {
-        acc.re += __shfl_xor(acc.re, off, SUB);
-        acc.im += __shfl_xor(acc.im, off, SUB);
+    for (int off = L / 2; off > 0; off >>= 1) {
+        acc.re += __shfl_xor(acc.re, off, L);
+        acc.im += __shfl_xor(acc.im, off, L);
     }
     if (lane == 0) {
         cx<R> v = keep ? acc : ld(x, row);
@@ -112,6 +111,30 @@ __global__ void __launch_bounds__(256) k_spmv(int64_t n, const int32_t *__restri
             v = cx<R>(bb.re - v.re, bb.im - v.im);
         }
         st(y, row, v);
+    }
+}
+
+// Lanes per CSR row for the SpMV: EMG_SPMV_LANES (4, 8, 16 or 32) if set, else 16.
+static int spmv_lanes() {
+    static int lanes = 0;
+    if (lanes == 0) {
+        const char *e = getenv("EMG_SPMV_LANES");
+        const int v = e ? atoi(e) : 16;
+        lanes = (v == 4 || v == 8 || v == 16 || v == 32) ? v : 16;
+    }
+    return lanes;
+}
+
+template <typename R>
+static void launch_spmv(hipStream_t st, int64_t n, const int32_t *rowptr, const int32_t *cols, const R *vals,
+                        const uint8_t *mask, const R *x, const R *b, R *y) {
+    const int L = spmv_lanes();
+    const unsigned grid = (unsigned)((n * L + 255) / 256);
+    switch (L) {
+        case 4: k_spmv<R, 4><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
+        case 8: k_spmv<R, 8><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
+        case 32: k_spmv<R, 32><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
+        default: k_spmv<R, 16><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
     }
 }
 
@@ -849,7 +872,7 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
     const int nbm = mdot_blocks(n, nv);
     const dim3 grid(nbm, (nv + DT - 1) / DT);
     apply_prec<R>(s, inv, vj, z, st);
-    k_spmv<R><<<spmv_blocks(n), 256, 0, st>>>(n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn);
+    launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn);
     k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, nullptr);
     k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, h1, 0, nullptr);
     k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, h1, vn, s->pn, nullptr);
@@ -898,7 +921,6 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     R *V = reinterpret_cast<R *>(s->V);
     R *w = reinterpret_cast<R *>(s->w);
     R *z = reinterpret_cast<R *>(s->z);
-    const unsigned spmv_grid = spmv_blocks(n);
     auto prec = [&](const R *x, R *y) { apply_prec<R>(s, inv, x, y, st); };
     double bnorm = 0;
     {
@@ -923,7 +945,7 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     double resid = 1.0;
     while (true) {
         // r = rhs - K u  ->  v0 = r / |r|
-        k_spmv<R><<<spmv_grid, 256, 0, st>>>(n, s->rowptr, s->cols, vals, s->mask, u, rhs, w);
+        launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, u, rhs, w);
         k_norm2<R><<<nb, RB, 0, st>>>(n, w, s->partial);
         k_scale_by_norm<R><<<nb, RB, 0, st>>>(n, s->partial, nb, w, V, s->nrm);
         double beta;
@@ -985,7 +1007,7 @@ extern "C" int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, d
     memset(info, 0, sizeof(*info));
     auto true_resid = [&](double *out_rel, double *bn) -> int {
         // r = b - K x in complex128 (masked)
-        k_spmv<double><<<spmv_blocks(n), 256, 0, st>>>(n, s->rowptr, s->cols, s->vals, s->mask, d_x, d_b, s->r);
+        launch_spmv<double>(st, n, s->rowptr, s->cols, s->vals, s->mask, d_x, d_b, s->r);
         k_norm2<double><<<nb, RB, 0, st>>>(n, s->r, s->partial);
         k_norm2<double><<<nb, RB, 0, st>>>(n, d_b, s->partial + 1024);
         std::vector<double> p(2048);
@@ -1079,8 +1101,7 @@ extern "C" int emg_scatter_add(int64_t m, const int32_t *d_pos, const double *d_
 extern "C" int emg_spmv(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_vals,
                         const uint8_t *d_mask, const double *d_x, double *d_y, void *stream) {
     EMG_CHECK(d_rowptr && d_cols && d_vals && d_x && d_y, "emg_spmv: null argument");
-    k_spmv<double><<<spmv_blocks(n), 256, 0, (hipStream_t)stream>>>(n, d_rowptr, d_cols, d_vals,
-                                                                                     d_mask, d_x, nullptr, d_y);
+    launch_spmv<double>((hipStream_t)stream, n, d_rowptr, d_cols, d_vals, d_mask, d_x, nullptr, d_y);
     EMG_LAUNCH_CHECK();
     return 0;
 }
