@@ -271,12 +271,49 @@ def bench_eigen(args):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     its = data.info["inner_iterations"]
+    lam = np.array([ds.eigenvalue for ds in data.datasets])
+    parity = None
+    if args.eigen_oracle:
+        # the oracle (C assembly + ARPACK shift-invert with PARDISO) in a child process; its shift sits
+        # amid the GPU's modes, above the curl-curl null space, so ARPACK returns the same physical modes
+        lam_o = eigen_oracle_child(float(np.mean(lam.real)), len(lam) + 6)
+        rel = [float(np.min(np.abs(lam_o - l)) / abs(l)) for l in lam]
+        parity = {"max_rel_diff": max(rel), "rel_diff": rel, "oracle": "C assembly + ARPACK shift-invert (PARDISO)"}
     return {"workload": "PEC cavity 30x20x15 mm, eps_r 6 block 10x10x5 mm 3 mm from a corner, 48x32x24 Kuhn hexes "
                         f"({mesh.n_tets:,} tets, {2 * mesh.n_entities:,} DOFs)", "modes": len(data),
             "seconds": dt, "modes_per_s": len(data) / dt, "target_hz": 5e9,
             "freqs_hz": [float(ds.freq) for ds in data.datasets],
             "relative_residuals": [float(ds.residual) for ds in data.datasets],
-            "inner_solves": len(its), "inner_iterations_mean": float(np.mean(its)) if its else 0.0}
+            "inner_solves": len(its), "inner_iterations_mean": float(np.mean(its)) if its else 0.0,
+            "parity_vs_oracle": parity}
+
+
+def eigen_oracle_child(sigma: float, k: int) -> np.ndarray:
+    import subprocess
+    env = dict(os.environ)
+    cores = str(min(len(os.sched_getaffinity(0)), 16))
+    env.update(OMP_NUM_THREADS=cores, MKL_NUM_THREADS=cores)
+    cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--eigen-oracle-child", "--eigen-sigma", str(sigma),
+                         "--eigen-k", str(k)], env=env, stdout=subprocess.PIPE, text=True)
+    if cp.returncode != 0:
+        raise RuntimeError(f"eigen oracle failed ({cp.returncode})")
+    v = json.loads(cp.stdout.strip().splitlines()[-1])
+    return np.array(v["re"]) + 1j * np.array(v["im"])
+
+
+def eigen_oracle(sigma: float, k: int) -> dict:
+    from emerge_amd import meshgen
+    from emerge_amd.bc import Material
+    from emerge_amd.physics import Electrodynamics3D
+    from oracle import fd
+    mesh = meshgen.cavity_block()
+    phys = Electrodynamics3D(mesh)
+    phys.set_material(meshgen.BLOCK_TAG, Material(er=6.0))
+    phys._bc_initialized = True
+    er, ur = phys.material_tensors()
+    orc = fd.OracleProblem(mesh, er, ur, phys.boundary_conditions)
+    w = fd.oracle_eigenmodes(orc, sigma, k, solver="pardiso", threads=min(len(os.sched_getaffinity(0)), 16), extra=0)
+    return {"re": w.real.tolist(), "im": w.imag.tolist()}
 
 
 def cpu_sweep_child(name: str, freq: float) -> dict:
@@ -371,6 +408,10 @@ def main():
                     help="comma-separated sweep variants to run: basis, plain, learned")
     ap.add_argument("--eigen", action="store_true", help="also run the cavity eigenmode workload")
     ap.add_argument("--eigen-modes", type=int, default=10)
+    ap.add_argument("--eigen-oracle", action="store_true", help="compare the eigenvalues with the CPU oracle (slow)")
+    ap.add_argument("--eigen-oracle-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--eigen-sigma", type=float, default=0.0, help=argparse.SUPPRESS)
+    ap.add_argument("--eigen-k", type=int, default=10, help=argparse.SUPPRESS)
     ap.add_argument("--sweep-kernel-only", action="store_true",
                     help="measure only the batched residual kernel of the sweep (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -378,6 +419,9 @@ def main():
     ap.add_argument("--cpu-sweep-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-sweep-freq", type=float, default=3e9, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.eigen_oracle_child:
+        print(json.dumps(eigen_oracle(args.eigen_sigma, args.eigen_k)), flush=True)
+        return
     if args.cpu_sweep_child:
         from emerge_amd.physics import Electrodynamics3D
         mesh, ports, _, desc = sweep_workload(args.cpu_sweep_child)

@@ -145,3 +145,16 @@ class EMSimData:
     def S_array(self) -> np.ndarray:
         """(nfreq, P, P) S matrices."""
         return np.stack([d.Sp.arry for d in self.datasets])
+
+    def export_touchstone(self, filename: str, format: str = "RI") -> str:
+        """Touchstone export of all frequency points (emdata.py:306-333); ports are
+        assumed numbered 1..N, as the reference assumes."""
+        from .touchstone import generate_touchstone
+        nports = len(self.datasets[0].Sp.map)
+        freqs = np.array([d.freq for d in self.datasets])
+        Smat = np.zeros((len(freqs), nports, nports), dtype=np.complex128)
+        for i in range(1, nports + 1):
+            for j in range(1, nports + 1):
+                _, S = self.ax("freq").S(i, j)
+                Smat[:, i - 1, j - 1] = S
+        return generate_touchstone(filename, freqs, Smat, data_format=format)

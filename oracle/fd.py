@@ -284,13 +284,37 @@ class OracleProblem:
         return S, sols
 
 
-def oracle_eigenmodes(problem: OracleProblem, sigma: float, k: int, spurious_rel: float = 1e-3):
+def oracle_eigenmodes(problem: OracleProblem, sigma: float, k: int, spurious_rel: float = 1e-3,
+                      solver: str = "superlu", threads: int | None = None, extra: int = 4):
     """Deprecated 3D eigen call of the reference (fem/physics/edm/emfreq3d.py:838-848):
     eigs(A, k, M=B, sigma, which='LM') on the PEC-reduced E and B (ARPACK with a
-    direct shift-invert factorisation); lambda ~ 0 (gradient) modes removed."""
+    direct shift-invert factorisation); lambda ~ 0 (gradient) modes removed.
+
+    solver="pardiso" factorises A - sigma B with MKL PARDISO (real nonsymmetric
+    when the materials are lossless, else the real block system) for large meshes."""
     _, _, solve_ids, _ = problem.assemble_freq_matrix(1.0)
     A = problem.E[solve_ids][:, solve_ids].tocsc()
     B = problem.B[solve_ids][:, solve_ids].tocsc()
-    w = spla.eigs(A, k=k + 4, M=B, sigma=sigma, which="LM", return_eigenvectors=False)
+    if solver == "pardiso":
+        from . import pardiso
+        S = (A - sigma * B).tocsr()
+        n = S.shape[0]
+        real = not np.any(S.data.imag) and not np.any(B.data.imag)
+        if real:
+            P = pardiso.Pardiso(sp.csr_matrix(S.real), threads=threads)
+            op = lambda x: P.solve(np.real(x)) + 1j * P.solve(np.imag(x)) if np.iscomplexobj(x) else P.solve(x)
+            A, B = A.real.tocsc(), B.real.tocsc()
+        else:
+            Sb, _ = pardiso.complex_to_real_block(S, np.zeros(n, dtype=np.complex128))
+            P = pardiso.Pardiso(Sb, threads=threads)
+
+            def op(x):
+                xr = P.solve(np.hstack([np.real(x), np.imag(x)]))
+                return xr[:n] + 1j * xr[n:]
+        OPinv = spla.LinearOperator(S.shape, matvec=op, dtype=np.complex128 if not real else np.float64)
+        w = spla.eigs(A, k=k + extra, M=B, sigma=sigma, which="LM", OPinv=OPinv, return_eigenvectors=False)
+        P.release()
+    else:
+        w = spla.eigs(A, k=k + extra, M=B, sigma=sigma, which="LM", return_eigenvectors=False)
     w = w[np.abs(w) > spurious_rel * abs(sigma)]
     return w[np.argsort(np.abs(w - sigma))][:k]
