@@ -62,6 +62,7 @@ struct emg_pattern {
     int fused_ok = 0;
     int32_t *chunk_tets = nullptr;   // nchunk * CHUNK_TMAX
     int32_t *chunk_ntets = nullptr;  // nchunk
+    int chunk_tmax = 0;              // largest chunk_ntets of this mesh (sizes the fused kernel's LDS)
     uint16_t *inc_local = nullptr;   // n_inc: (local tet << 8) | (s << 4) | s'
     uint8_t *recipes = nullptr;      // 100 * RECIPE_BYTES
 };
@@ -262,9 +263,13 @@ extern "C" int emg_pattern_build(const int32_t *d_tet_ent, int64_t nt, int64_t n
         TRY(hipGetLastError());
         int over = 0;
         TRY(hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, st));
+        std::vector<int32_t> cnt((size_t)pat->nchunk);
+        TRY(hipMemcpyAsync(cnt.data(), pat->chunk_ntets, cnt.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         TRY(hipStreamSynchronize(st));
         hfree(d_over);
         pat->fused_ok = over ? 0 : 1;
+        pat->chunk_tmax = 0;
+        for (int32_t v : cnt) pat->chunk_tmax = std::max(pat->chunk_tmax, (int)v);
     }
     TRY(hipStreamSynchronize(st));
     hfree(uniq);
@@ -837,12 +842,16 @@ static constexpr int TD_FIELDS = 41;  // H(21) G(10) L(6) eps(2) nu(2)
 
 __device__ __forceinline__ uint32_t rbyte(const uint32_t *w, int i) { return (w[i >> 2] >> ((i & 3) * 8)) & 255u; }
 
+// TS: LDS stride of the per-tet Gram data = the largest number of distinct tets
+// of a chunk, rounded up (32 / 48 / 64 / 128): sized to the mesh, the LDS of a
+// workgroup shrinks from 47 KB to ~20 KB and more workgroups fit on a CU.
+template <int TS>
 __global__ void __launch_bounds__(256) k_assemble_fused(const emg_pattern pat, const double *__restrict__ nodes,
                                                         const int32_t *__restrict__ tets, const cplx *__restrict__ eps,
                                                         const cplx *__restrict__ nu, int exact_face_mass,
                                                         cplx *__restrict__ E, cplx *__restrict__ B) {
     __shared__ uint32_t rcp[100 * RECIPE_BYTES / 4];
-    __shared__ double td[TD_FIELDS * CHUNK_TMAX];
+    __shared__ double td[TD_FIELDS * TS];
     const int tid = threadIdx.x;
     for (int i = tid; i < 100 * RECIPE_BYTES / 4; i += 256) rcp[i] = reinterpret_cast<const uint32_t *>(pat.recipes)[i];
     for (int64_t c = blockIdx.x; c < pat.nchunk; c += gridDim.x) {
@@ -870,21 +879,21 @@ __global__ void __launch_bounds__(256) k_assemble_fused(const emg_pattern pat, c
 #pragma unroll
             for (int k = 0; k < 6; k++) {
                 X[k] = crs(gr[EU[k]], gr[EV[k]]);
-                td[(31 + k) * CHUNK_TMAX + tid] = nrm(sub(P[EV[k]], P[EU[k]]));
+                td[(31 + k) * TS + tid] = nrm(sub(P[EV[k]], P[EU[k]]));
             }
 #pragma unroll
             for (int k = 0; k < 6; k++)
 #pragma unroll
-                for (int l = k; l < 6; l++) td[sym6(k, l) * CHUNK_TMAX + tid] = dot(X[k], X[l]) * dscale;
+                for (int l = k; l < 6; l++) td[sym6(k, l) * TS + tid] = dot(X[k], X[l]) * dscale;
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = a; b < 4; b++) td[(21 + sym4(a, b)) * CHUNK_TMAX + tid] = dot(gr[a], gr[b]) * fscale;
+                for (int b = a; b < 4; b++) td[(21 + sym4(a, b)) * TS + tid] = dot(gr[a], gr[b]) * fscale;
             const cplx ev = eps[t], nv = nu[t];
-            td[37 * CHUNK_TMAX + tid] = ev.re;
-            td[38 * CHUNK_TMAX + tid] = ev.im;
-            td[39 * CHUNK_TMAX + tid] = nv.re;
-            td[40 * CHUNK_TMAX + tid] = nv.im;
+            td[37 * TS + tid] = ev.re;
+            td[38 * TS + tid] = ev.im;
+            td[39 * TS + tid] = nv.re;
+            td[40 * TS + tid] = nv.im;
         }
         __syncthreads();
         const int64_t k = c * CHUNK + tid;
@@ -912,9 +921,9 @@ __global__ void __launch_bounds__(256) k_assemble_fused(const emg_pattern pat, c
             for (int x = 0; x < 3; x++)
 #pragma unroll
                 for (int y = 0; y < 3; y++) {
-                    const double hv = T[rbyte(w, x * 3 + y) * CHUNK_TMAX];
+                    const double hv = T[rbyte(w, x * 3 + y) * TS];
                     h[x][y] = ((wm >> (x * 3 + y)) & 1u) ? 2.0 * hv : hv;
-                    g[x][y] = T[(21 + rbyte(w, 9 + x * 3 + y)) * CHUNK_TMAX];
+                    g[x][y] = T[(21 + rbyte(w, 9 + x * 3 + y)) * TS];
                 }
             const uint32_t fl = rbyte(w, 44);
             const bool fi = fl & 1u, fj = fl & 2u;
@@ -922,9 +931,9 @@ __global__ void __launch_bounds__(256) k_assemble_fused(const emg_pattern pat, c
                                       {fi ? -2.0 : 0.0, fi ? -1.0 : -3.0, fi ? 1.0 : 0.0}};
             const double cfj[2][3] = {{fj ? -1.0 : -3.0, fj ? -2.0 : 0.0, fj ? -1.0 : 0.0},
                                       {fj ? -2.0 : 0.0, fj ? -1.0 : -3.0, fj ? 1.0 : 0.0}};
-            const double si[2] = {(fi ? -1.0 : 1.0) * T[(31 + rbyte(w, 40)) * CHUNK_TMAX], T[(31 + rbyte(w, 41)) * CHUNK_TMAX]};
-            const double sj[2] = {(fj ? -1.0 : 1.0) * T[(31 + rbyte(w, 42)) * CHUNK_TMAX], T[(31 + rbyte(w, 43)) * CHUNK_TMAX]};
-            const cplx ct(T[37 * CHUNK_TMAX], T[38 * CHUNK_TMAX]), nt_(T[39 * CHUNK_TMAX], T[40 * CHUNK_TMAX]);
+            const double si[2] = {(fi ? -1.0 : 1.0) * T[(31 + rbyte(w, 40)) * TS], T[(31 + rbyte(w, 41)) * TS]};
+            const double sj[2] = {(fj ? -1.0 : 1.0) * T[(31 + rbyte(w, 42)) * TS], T[(31 + rbyte(w, 43)) * TS]};
+            const cplx ct(T[37 * TS], T[38 * TS]), nt_(T[39 * TS], T[40 * TS]);
             double hc[3][2];
 #pragma unroll
             for (int x = 0; x < 3; x++)
@@ -989,10 +998,18 @@ extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, c
             EMG_HIP(hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev));
         }
         const int64_t grid = std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * 8);
-        k_assemble_fused<<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, reinterpret_cast<const cplx *>(d_eps),
-                                                         reinterpret_cast<const cplx *>(d_nu),
-                                                         (flags & EMG_EXACT_FACE_MASS) ? 1 : 0,
-                                                         reinterpret_cast<cplx *>(d_E), reinterpret_cast<cplx *>(d_B));
+        const int efm = (flags & EMG_EXACT_FACE_MASS) ? 1 : 0;
+        const cplx *ce = reinterpret_cast<const cplx *>(d_eps), *cn = reinterpret_cast<const cplx *>(d_nu);
+        cplx *oE = reinterpret_cast<cplx *>(d_E), *oB = reinterpret_cast<cplx *>(d_B);
+        const int tm = pat->chunk_tmax;
+        if (tm <= 32)
+            k_assemble_fused<32><<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, ce, cn, efm, oE, oB);
+        else if (tm <= 48)
+            k_assemble_fused<48><<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, ce, cn, efm, oE, oB);
+        else if (tm <= 64)
+            k_assemble_fused<64><<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, ce, cn, efm, oE, oB);
+        else
+            k_assemble_fused<CHUNK_TMAX><<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, ce, cn, efm, oE, oB);
         EMG_LAUNCH_CHECK();
         return 0;
     }
