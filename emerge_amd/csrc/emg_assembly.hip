@@ -59,11 +59,15 @@ struct emg_pattern {
     double *tet_blocks = nullptr; // nt * TET_BLOCK_DOUBLES, two-phase fallback workspace
     // fused path: per chunk of CHUNK adjacency entries, its sorted distinct tets
     int64_t nchunk = 0;
+    int chunk = 0;                   // adjacency entries per chunk (CHUNK)
     int fused_ok = 0;
+    int32_t *chunk_row0 = nullptr;   // nchunk + 1, row of each chunk's first entry (last: n_ent - 1)
     int32_t *chunk_tets = nullptr;   // nchunk * CHUNK_TMAX
     int32_t *chunk_ntets = nullptr;  // nchunk
     int chunk_tmax = 0;              // largest chunk_ntets of this mesh (sizes the fused kernel's LDS)
     uint16_t *inc_local = nullptr;   // n_inc: (local tet << 8) | (s << 4) | s'
+    uint8_t *chunk_perm = nullptr;   // nchunk * CHUNK: chunk-local entries by descending incidence count
+    double *tet_rec = nullptr;       // nt * REC: per-tet Gram records (fused-path workspace)
     uint8_t *recipes = nullptr;      // 100 * RECIPE_BYTES
 };
 
@@ -73,8 +77,8 @@ static constexpr int PAIR_BLOCK = 256;
 // mass is not symmetric there), each block (m,m') row-major.
 static constexpr int N_UPPER = 55;
 static constexpr int TET_BLOCK_DOUBLES = 4 * (N_UPPER + N_UPPER + 6);
-// Fused path: a workgroup owns CHUNK consecutive adjacency entries and stages
-// the Gram data of their (at most CHUNK_TMAX) distinct tets in LDS.
+// Fused path: a workgroup owns `chunk` (at most CHUNK) consecutive adjacency
+// entries and stages the Gram data of their (at most CHUNK_TMAX) distinct tets in LDS.
 static constexpr int CHUNK = 256;
 static constexpr int CHUNK_TMAX = 128;
 static constexpr int CHUNK_NMAX = 1024;  // incidences per chunk handled by the tet-list builder
@@ -83,7 +87,7 @@ static constexpr int RECIPE_BYTES = 48;
 static void build_recipes(uint8_t *out);
 __global__ void __launch_bounds__(256) k_chunk_tets(const emg_pattern pat, int32_t *__restrict__ chunk_tets,
                                                     int32_t *__restrict__ chunk_ntets, uint16_t *__restrict__ inc_local,
-                                                    int *__restrict__ overflow);
+                                                    uint8_t *__restrict__ chunk_perm, int *__restrict__ overflow);
 
 // ------------------------------------------------------------------ symbolic
 __global__ void k_make_keys(const int32_t *__restrict__ tet_ent, int64_t nt, int bits,
@@ -115,10 +119,10 @@ __global__ void k_rows_from_keys(const uint64_t *__restrict__ uniq, int64_t nnz_
 }
 
 __global__ void k_block_rows(const uint64_t *__restrict__ uniq, int64_t nnz_ent, int bits, int64_t nblk,
-                             int64_t n_ent, int32_t *__restrict__ blk_row0) {
+                             int64_t n_ent, int block, int32_t *__restrict__ blk_row0) {
     int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (b > nblk) return;
-    blk_row0[b] = (b == nblk) ? (int32_t)(n_ent - 1) : (int32_t)(uniq[b * PAIR_BLOCK] >> bits);
+    blk_row0[b] = (b == nblk) ? (int32_t)(n_ent - 1) : (int32_t)(uniq[b * (int64_t)block] >> bits);
 }
 
 __global__ void k_set_last(int32_t *p, int64_t idx, int32_t v) { p[idx] = v; }
@@ -243,23 +247,30 @@ extern "C" int emg_pattern_build(const int32_t *d_tet_ent, int64_t nt, int64_t n
     }
     k_set_last<<<1, 1, 0, st>>>(pat->pair_ptr, nnz_ent, (int32_t)n_inc);
     k_rows_from_keys<<<emg::grid_for(nnz_ent + 1, 256), 256, 0, st>>>(uniq, nnz_ent, bits, n_ent, pat->ent_ptr, pat->adj);
-    k_block_rows<<<emg::grid_for(pat->nblk + 1, 256), 256, 0, st>>>(uniq, nnz_ent, bits, pat->nblk, n_ent, pat->blk_row0);
+    k_block_rows<<<emg::grid_for(pat->nblk + 1, 256), 256, 0, st>>>(uniq, nnz_ent, bits, pat->nblk, n_ent, PAIR_BLOCK,
+                                                                   pat->blk_row0);
     TRY(hipGetLastError());
-    // fused-path chunk tables (chunks coincide with the PAIR_BLOCK row blocks)
-    static_assert(CHUNK == PAIR_BLOCK, "chunks reuse the row-block table");
-    pat->nchunk = pat->nblk;
+    // fused-path chunk tables
+    pat->chunk = CHUNK;
+    pat->nchunk = (nnz_ent + pat->chunk - 1) / pat->chunk;
+    TRY(hipMalloc(&pat->chunk_row0, (pat->nchunk + 1) * sizeof(int32_t)));
+    k_block_rows<<<emg::grid_for(pat->nchunk + 1, 256), 256, 0, st>>>(uniq, nnz_ent, bits, pat->nchunk, n_ent, pat->chunk,
+                                                                     pat->chunk_row0);
     {
         int *d_over = nullptr;
         TRY(hipMalloc(&pat->chunk_tets, (size_t)pat->nchunk * CHUNK_TMAX * sizeof(int32_t)));
         TRY(hipMalloc(&pat->chunk_ntets, (size_t)pat->nchunk * sizeof(int32_t)));
-        TRY(hipMalloc(&pat->inc_local, (size_t)n_inc * sizeof(uint16_t)));
+        // two spare codes: the fused kernel reads the codes as 32-bit words
+        TRY(hipMalloc(&pat->inc_local, (size_t)(n_inc + 2) * sizeof(uint16_t)));
+        TRY(hipMalloc(&pat->chunk_perm, (size_t)pat->nchunk * pat->chunk));
         TRY(hipMalloc(&pat->recipes, 100 * RECIPE_BYTES));
         TRY(hipMalloc(&d_over, sizeof(int)));
         TRY(hipMemsetAsync(d_over, 0, sizeof(int), st));
         uint8_t host_rec[100 * RECIPE_BYTES];
         build_recipes(host_rec);
         TRY(hipMemcpyAsync(pat->recipes, host_rec, sizeof(host_rec), hipMemcpyHostToDevice, st));
-        k_chunk_tets<<<(unsigned)pat->nchunk, 256, 0, st>>>(*pat, pat->chunk_tets, pat->chunk_ntets, pat->inc_local, d_over);
+        k_chunk_tets<<<(unsigned)pat->nchunk, 256, 0, st>>>(*pat, pat->chunk_tets, pat->chunk_ntets, pat->inc_local,
+                                                             pat->chunk_perm, d_over);
         TRY(hipGetLastError());
         int over = 0;
         TRY(hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -311,6 +322,9 @@ extern "C" void emg_pattern_destroy(emg_pattern *pat) {
     hfree(pat->chunk_tets);
     hfree(pat->chunk_ntets);
     hfree(pat->inc_local);
+    hfree(pat->chunk_perm);
+    hfree(pat->chunk_row0);
+    hfree(pat->tet_rec);
     hfree(pat->recipes);
     delete pat;
 }
@@ -785,16 +799,39 @@ static void build_recipes(uint8_t *out) {
 // Distinct tets of every chunk, and chunk-local incidence codes.
 __global__ void __launch_bounds__(256) k_chunk_tets(const emg_pattern pat, int32_t *__restrict__ chunk_tets,
                                                     int32_t *__restrict__ chunk_ntets, uint16_t *__restrict__ inc_local,
-                                                    int *__restrict__ overflow) {
+                                                    uint8_t *__restrict__ chunk_perm, int *__restrict__ overflow) {
     __shared__ uint32_t key[CHUNK_NMAX];
     __shared__ uint32_t uq[CHUNK_NMAX];
+    __shared__ uint8_t ckey[CHUNK];
     __shared__ int nuniq;
     const int64_t c = blockIdx.x;
     const int tid = threadIdx.x;
-    const int64_t k0 = c * CHUNK, k1 = (k0 + CHUNK < pat.nnz_ent) ? k0 + CHUNK : pat.nnz_ent;
+    const int ch = pat.chunk;
+    const int64_t k0 = c * ch, k1 = (k0 + ch < pat.nnz_ent) ? k0 + ch : pat.nnz_ent;
     const int32_t q0 = pat.pair_ptr[k0], q1 = pat.pair_ptr[k1];
     const int n = q1 - q0;
-    if (n > CHUNK_NMAX) {
+    // Lane order of the fused kernel: the chunk's entries sorted (stably) by
+    // descending incidence count, so a wavefront's lanes loop a similar number
+    // of times.  Position r of the chunk holds local entry chunk_perm[k0 + r].
+    const int np = (int)(k1 - k0);
+    int myk = 255;
+    if (tid < np) {
+        const int cnt = pat.pair_ptr[k0 + tid + 1] - pat.pair_ptr[k0 + tid];
+        myk = 255 - (cnt < 255 ? cnt : 255);
+    }
+    ckey[tid] = (uint8_t)myk;
+    __syncthreads();
+    if (tid < np) {
+        int rank = 0;
+        for (int j = 0; j < np; j++) {
+            const int kj = ckey[j];
+            rank += (kj < myk || (kj == myk && j < tid)) ? 1 : 0;
+        }
+        chunk_perm[k0 + rank] = (uint8_t)tid;
+    } else if (tid < ch) {
+        chunk_perm[k0 + tid] = (uint8_t)tid;
+    }
+    if (n > CHUNK_NMAX * ch / CHUNK) {   // the fused kernel stages at most this many incidences
         if (tid == 0) { atomicOr(overflow, 1); chunk_ntets[c] = -1; }
         return;
     }
@@ -842,133 +879,315 @@ static constexpr int TD_FIELDS = 41;  // H(21) G(10) L(6) eps(2) nu(2)
 
 __device__ __forceinline__ uint32_t rbyte(const uint32_t *w, int i) { return (w[i >> 2] >> ((i & 3) * 8)) & 255u; }
 
-// TS: LDS stride of the per-tet Gram data = the largest number of distinct tets
-// of a chunk, rounded up (32 / 48 / 64 / 128): sized to the mesh, the LDS of a
-// workgroup shrinks from 47 KB to ~20 KB and more workgroups fit on a CU.
+// Gram record of a tet (vertices P, materials ev, nv): field f at dst[f * S].
+// Fields: H = (grad x grad) products of the 6 edges (0..20), G = grad.grad of
+// the 4 vertices (21..30), the 6 edge lengths (31..36), eps (37, 38), nu (39, 40).
+template <int S>
+__device__ __forceinline__ void fused_tet_gram(const V3 (&P)[4], cplx ev, cplx nv, double *dst) {
+    const V3 e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]), e3 = sub(P[3], P[0]);
+    V3 gr[4];
+    gr[1] = crs(e2, e3);
+    gr[2] = crs(e3, e1);
+    gr[3] = crs(e1, e2);
+    gr[0] = {-(gr[1].x + gr[2].x + gr[3].x), -(gr[1].y + gr[2].y + gr[3].y), -(gr[1].z + gr[2].z + gr[3].z)};
+    const double V6 = fabs(dot(e1, gr[1]));
+    const double fscale = 1.0 / (5040.0 * V6);
+    const double dscale = 1.0 / (120.0 * V6 * V6 * V6);
+    constexpr int EU[6] = {0, 0, 0, 1, 1, 2}, EV[6] = {1, 2, 3, 2, 3, 3};
+#pragma unroll
+    for (int k = 0; k < 6; k++) dst[(31 + k) * S] = nrm(sub(P[EV[k]], P[EU[k]]));
+    // H_kl = (g_a x g_b).(g_c x g_d) = G_ac G_bd - G_ad G_bc (Binet-Cauchy) with
+    // G = the Gram matrix of the vertex gradients: no edge cross products held.
+    double G[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = a; b < 4; b++) {
+            G[a][b] = dot(gr[a], gr[b]);
+            G[b][a] = G[a][b];
+            dst[(21 + sym4(a, b)) * S] = G[a][b] * fscale;
+        }
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+#pragma unroll
+        for (int l = k; l < 6; l++) {
+            const int a = EU[k], b = EV[k], c = EU[l], d = EV[l];
+            dst[sym6(k, l) * S] = (G[a][c] * G[b][d] - G[a][d] * G[b][c]) * dscale;
+        }
+    dst[37 * S] = ev.re;
+    dst[38 * S] = ev.im;
+    dst[39 * S] = nv.re;
+    dst[40 * S] = nv.im;
+}
+
+// One incidence v = (local tet << 8) | (s << 4) | s' of an adjacency entry:
+// adds nu_t * D_t and eps_t * F_t of the slot pair (s, s') (2x2 each) into aD, aF.
 template <int TS>
-__global__ void __launch_bounds__(256) k_assemble_fused(const emg_pattern pat, const double *__restrict__ nodes,
-                                                        const int32_t *__restrict__ tets, const cplx *__restrict__ eps,
-                                                        const cplx *__restrict__ nu, int exact_face_mass,
-                                                        cplx *__restrict__ E, cplx *__restrict__ B) {
-    __shared__ uint32_t rcp[100 * RECIPE_BYTES / 4];
-    __shared__ double td[TD_FIELDS * TS];
-    const int tid = threadIdx.x;
-    for (int i = tid; i < 100 * RECIPE_BYTES / 4; i += 256) rcp[i] = reinterpret_cast<const uint32_t *>(pat.recipes)[i];
-    for (int64_t c = blockIdx.x; c < pat.nchunk; c += gridDim.x) {
-        __syncthreads();
-        const int nT = pat.chunk_ntets[c];
-        if (tid < nT) {
-            const int t = pat.chunk_tets[c * CHUNK_TMAX + tid];
-            const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
-            const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
-            V3 P[4];
+__device__ __forceinline__ void fused_incidence(const uint32_t *rcp, const double *td, uint32_t v,
+                                                int exact_face_mass, cplx (&aD)[2][2], cplx (&aF)[2][2]) {
+    const int tl = (int)(v >> 8);
+    const int sp = (int)(((v >> 4) & 15u) * 10 + (v & 15u));
+    uint32_t w[12];
+    const uint4 *R4 = reinterpret_cast<const uint4 *>(rcp + sp * (RECIPE_BYTES / 4));
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                P[i] = {nodes[3 * (int64_t)vid[i]], nodes[3 * (int64_t)vid[i] + 1], nodes[3 * (int64_t)vid[i] + 2]};
-            const V3 e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]), e3 = sub(P[3], P[0]);
-            V3 gr[4];
-            gr[1] = crs(e2, e3);
-            gr[2] = crs(e3, e1);
-            gr[3] = crs(e1, e2);
-            gr[0] = {-(gr[1].x + gr[2].x + gr[3].x), -(gr[1].y + gr[2].y + gr[3].y), -(gr[1].z + gr[2].z + gr[3].z)};
-            const double V6 = fabs(dot(e1, gr[1]));
-            const double fscale = 1.0 / (5040.0 * V6);
-            const double dscale = 1.0 / (120.0 * V6 * V6 * V6);
-            constexpr int EU[6] = {0, 0, 0, 1, 1, 2}, EV[6] = {1, 2, 3, 2, 3, 3};
-            V3 X[6];
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                X[k] = crs(gr[EU[k]], gr[EV[k]]);
-                td[(31 + k) * TS + tid] = nrm(sub(P[EV[k]], P[EU[k]]));
-            }
-#pragma unroll
-            for (int k = 0; k < 6; k++)
-#pragma unroll
-                for (int l = k; l < 6; l++) td[sym6(k, l) * TS + tid] = dot(X[k], X[l]) * dscale;
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = a; b < 4; b++) td[(21 + sym4(a, b)) * TS + tid] = dot(gr[a], gr[b]) * fscale;
-            const cplx ev = eps[t], nv = nu[t];
-            td[37 * TS + tid] = ev.re;
-            td[38 * TS + tid] = ev.im;
-            td[39 * TS + tid] = nv.re;
-            td[40 * TS + tid] = nv.im;
-        }
-        __syncthreads();
-        const int64_t k = c * CHUNK + tid;
-        if (k >= pat.nnz_ent) continue;
-        const int e = find_row(pat.ent_ptr, pat.blk_row0, k);
-        const int32_t p0 = pat.ent_ptr[e];
-        const int32_t deg = pat.ent_ptr[e + 1] - p0;
-        cplx aD[2][2], aF[2][2];
-        const int32_t q0 = pat.pair_ptr[k], q1 = pat.pair_ptr[k + 1];
-        for (int32_t q = q0; q < q1; q++) {
-            const uint32_t v = pat.inc_local[q];
-            const int tl = (int)(v >> 8);
-            const int sp = (int)(((v >> 4) & 15u) * 10 + (v & 15u));
-            uint32_t w[12];
-            const uint4 *R4 = reinterpret_cast<const uint4 *>(rcp + sp * (RECIPE_BYTES / 4));
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const uint4 u = R4[i];
-                w[4 * i] = u.x; w[4 * i + 1] = u.y; w[4 * i + 2] = u.z; w[4 * i + 3] = u.w;
-            }
-            const double *T = td + tl;
-            const uint32_t wm = rbyte(w, 18) | (rbyte(w, 19) << 8);
-            double h[3][3], g[3][3];
-#pragma unroll
-            for (int x = 0; x < 3; x++)
-#pragma unroll
-                for (int y = 0; y < 3; y++) {
-                    const double hv = T[rbyte(w, x * 3 + y) * TS];
-                    h[x][y] = ((wm >> (x * 3 + y)) & 1u) ? 2.0 * hv : hv;
-                    g[x][y] = T[(21 + rbyte(w, 9 + x * 3 + y)) * TS];
-                }
-            const uint32_t fl = rbyte(w, 44);
-            const bool fi = fl & 1u, fj = fl & 2u;
-            const double cfi[2][3] = {{fi ? -1.0 : -3.0, fi ? -2.0 : 0.0, fi ? -1.0 : 0.0},
-                                      {fi ? -2.0 : 0.0, fi ? -1.0 : -3.0, fi ? 1.0 : 0.0}};
-            const double cfj[2][3] = {{fj ? -1.0 : -3.0, fj ? -2.0 : 0.0, fj ? -1.0 : 0.0},
-                                      {fj ? -2.0 : 0.0, fj ? -1.0 : -3.0, fj ? 1.0 : 0.0}};
-            const double si[2] = {(fi ? -1.0 : 1.0) * T[(31 + rbyte(w, 40)) * TS], T[(31 + rbyte(w, 41)) * TS]};
-            const double sj[2] = {(fj ? -1.0 : 1.0) * T[(31 + rbyte(w, 42)) * TS], T[(31 + rbyte(w, 43)) * TS]};
-            const cplx ct(T[37 * TS], T[38 * TS]), nt_(T[39 * TS], T[40 * TS]);
-            double hc[3][2];
-#pragma unroll
-            for (int x = 0; x < 3; x++)
-#pragma unroll
-                for (int m2 = 0; m2 < 2; m2++)
-                    hc[x][m2] = h[x][0] * cfj[m2][0] + h[x][1] * cfj[m2][1] + h[x][2] * cfj[m2][2];
-#pragma unroll
-            for (int m = 0; m < 2; m++) {
-                const bool qc = (m == 0) && fi;
-                const double g_q0 = qc ? g[2][0] : g[1][0];
-#pragma unroll
-                for (int m2 = 0; m2 < 2; m2++) {
-                    const bool qc2 = (m2 == 0) && fj;
-                    const double g_0q = qc2 ? g[0][2] : g[0][1];
-                    const double g_qq = qc ? (qc2 ? g[2][2] : g[2][1]) : (qc2 ? g[1][2] : g[1][1]);
-                    const int o = 20 + 4 * (m * 2 + m2);
-                    const double c1 = (double)(exact_face_mass ? rbyte(w, o + 1) : rbyte(w, 36 + m * 2 + m2));
-                    const double f = (double)rbyte(w, o) * g[0][0] - c1 * g_0q - (double)rbyte(w, o + 2) * g_q0 +
-                                     (double)rbyte(w, o + 3) * g_qq;
-                    const double d = cfi[m][0] * hc[0][m2] + cfi[m][1] * hc[1][m2] + cfi[m][2] * hc[2][m2];
-                    const double sij = si[m] * sj[m2];
-                    aD[m][m2] += nt_ * (sij * d);
-                    aF[m][m2] += ct * (sij * f);
-                }
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < 2; m++)
-#pragma unroll
-            for (int m2 = 0; m2 < 2; m2++) {
-                const int64_t idx = (int64_t)m * 2 * pat.nnz_ent + p0 + (int64_t)m2 * deg + k;
-                E[idx] = aD[m][m2];
-                B[idx] = aF[m][m2];
-            }
+    for (int i = 0; i < 3; i++) {
+        const uint4 u = R4[i];
+        w[4 * i] = u.x; w[4 * i + 1] = u.y; w[4 * i + 2] = u.z; w[4 * i + 3] = u.w;
     }
+    const double *T = td + tl;
+    const uint32_t wm = rbyte(w, 18) | (rbyte(w, 19) << 8);
+    double h[3][3], g[3][3];
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 3; y++) {
+            const double hv = T[rbyte(w, x * 3 + y) * TS];
+            h[x][y] = ((wm >> (x * 3 + y)) & 1u) ? 2.0 * hv : hv;
+            g[x][y] = T[(21 + rbyte(w, 9 + x * 3 + y)) * TS];
+        }
+    const uint32_t fl = rbyte(w, 44);
+    const bool fi = fl & 1u, fj = fl & 2u;
+    const double cfi[2][3] = {{fi ? -1.0 : -3.0, fi ? -2.0 : 0.0, fi ? -1.0 : 0.0},
+                              {fi ? -2.0 : 0.0, fi ? -1.0 : -3.0, fi ? 1.0 : 0.0}};
+    const double cfj[2][3] = {{fj ? -1.0 : -3.0, fj ? -2.0 : 0.0, fj ? -1.0 : 0.0},
+                              {fj ? -2.0 : 0.0, fj ? -1.0 : -3.0, fj ? 1.0 : 0.0}};
+    const double si[2] = {(fi ? -1.0 : 1.0) * T[(31 + rbyte(w, 40)) * TS], T[(31 + rbyte(w, 41)) * TS]};
+    const double sj[2] = {(fj ? -1.0 : 1.0) * T[(31 + rbyte(w, 42)) * TS], T[(31 + rbyte(w, 43)) * TS]};
+    const cplx ct(T[37 * TS], T[38 * TS]), nt_(T[39 * TS], T[40 * TS]);
+    double hc[3][2];
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int m2 = 0; m2 < 2; m2++)
+            hc[x][m2] = h[x][0] * cfj[m2][0] + h[x][1] * cfj[m2][1] + h[x][2] * cfj[m2][2];
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        const bool qc = (m == 0) && fi;
+        const double g_q0 = qc ? g[2][0] : g[1][0];
+#pragma unroll
+        for (int m2 = 0; m2 < 2; m2++) {
+            const bool qc2 = (m2 == 0) && fj;
+            const double g_0q = qc2 ? g[0][2] : g[0][1];
+            const double g_qq = qc ? (qc2 ? g[2][2] : g[2][1]) : (qc2 ? g[1][2] : g[1][1]);
+            const int o = 20 + 4 * (m * 2 + m2);
+            const double c1 = (double)(exact_face_mass ? rbyte(w, o + 1) : rbyte(w, 36 + m * 2 + m2));
+            const double f = (double)rbyte(w, o) * g[0][0] - c1 * g_0q - (double)rbyte(w, o + 2) * g_q0 +
+                             (double)rbyte(w, o + 3) * g_qq;
+            const double d = cfi[m][0] * hc[0][m2] + cfi[m][1] * hc[1][m2] + cfi[m][2] * hc[2][m2];
+            const double sij = si[m] * sj[m2];
+            aD[m][m2] += nt_ * (sij * d);
+            aF[m][m2] += ct * (sij * f);
+        }
+    }
+}
+
+// Gram record of every tet (REC doubles each, fields as fused_tet_gram): the
+// first step of the numeric assembly, so each tet's geometry is computed once.
+static constexpr int REC = 48;
+__global__ void __launch_bounds__(256) k_tet_gram(int64_t nt, const double *__restrict__ nodes,
+                                                  const int32_t *__restrict__ tets, const cplx *__restrict__ eps,
+                                                  const cplx *__restrict__ nu, double *__restrict__ rec) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
+    const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+    V3 P[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        P[j] = {nodes[3 * (int64_t)vid[j]], nodes[3 * (int64_t)vid[j] + 1], nodes[3 * (int64_t)vid[j] + 2]};
+    fused_tet_gram<1>(P, eps[t], nu[t], rec + t * REC);
+}
+
+// Numeric assembly, one workgroup of CH lanes per chunk of CH consecutive
+// adjacency entries (grid-stride over chunks).  Per chunk:
+//  * its index data (entry offsets into the incidences, the offsets of the rows
+//    it spans, the incidence codes, the lane order) and the Gram records of its
+//    distinct tets (k_tet_gram) are loaded into registers while the previous
+//    chunk writes its output (the tet ids one step earlier, while it runs its
+//    incidence loop), and put into LDS when the chunk starts: no global load is
+//    waited for on the chunk's critical path, and the incidence loop reads only LDS;
+//  * lane r accumulates entry chunk_perm[r] (entries by descending incidence
+//    count, so the lanes of a wavefront loop a similar number of times);
+//  * the 2x2 blocks go back through LDS (reusing the Gram area) and are written
+//    in entry order, coalesced.
+// The wavefront taking the heaviest entries rotates with the chunk, so no SIMD
+// always gets them.  Launch bound: 4 waves per SIMD (3 for the widest LDS layout).
+template <int TS, int CH>
+__global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
+    k_assemble_staged(const emg_pattern pat, const double *__restrict__ rec, int exact_face_mass,
+                      cplx *__restrict__ E, cplx *__restrict__ B) {
+    constexpr int TSP = TS + 1;             // odd field stride of the Gram data: fewer LDS bank conflicts
+    constexpr int TD = TD_FIELDS * TSP;     // doubles of Gram data
+    constexpr int SD = 2 * 4 * CH;          // doubles of one staged matrix: 4 blocks x CH complex
+    constexpr int NINC = CHUNK_NMAX * CH / CHUNK;
+    constexpr int NID = NINC / 2 + 1;       // incidence codes as 32-bit words (first one may be half used)
+    constexpr int RI = (NID + CH - 1) / CH; // words per lane
+    constexpr int NW = CH / 64;
+    constexpr int SPR = CH / 8;             // tet records per round: 8 lanes x 6 doubles each
+    constexpr int RR = (TS + SPR - 1) / SPR;
+    static_assert(8 * 6 >= TD_FIELDS && 6 * 8 <= REC, "8 lanes of 6 doubles cover a record");
+    __shared__ uint32_t rcp[100 * RECIPE_BYTES / 4];
+    __shared__ __align__(16) double td[TD > SD ? TD : SD];
+    __shared__ int32_t s_pp[CH + 1];        // entry -> first incidence (absolute)
+    __shared__ int32_t s_row[CH + 2];       // ent_ptr of the rows the chunk spans
+    __shared__ uint32_t s_inc[NID];         // the chunk's incidence codes (16-bit), from an even index
+    __shared__ uint8_t s_perm[CH];          // lane order
+    __shared__ int2 s_out[CH];              // per entry: (row start, row length)
+    const int tid = threadIdx.x;
+    const int rs = tid >> 3, rsub = tid & 7;   // record slot within a round, 6-double part of the record
+    for (int i = tid; i < 100 * RECIPE_BYTES / 4; i += CH) rcp[i] = reinterpret_cast<const uint32_t *>(pat.recipes)[i];
+    cplx *stg = reinterpret_cast<cplx *>(td);
+    // registers holding the next chunk's data; every one is assigned on every
+    // path, so none stays live through the incidence loop
+    int32_t f_q0 = 0, f_ninc = 0, f_nr = 0, f_nt = 0, g_nt = 0;   // uniform
+    int32_t f_pp[2], f_row[2], g_t[RR];
+    uint32_t f_inc[RI];
+    int f_perm = 0;
+    double2 f_rec[RR][3];
+    auto tet_ids = [&](int64_t cc) {
+        g_nt = cc < pat.nchunk ? pat.chunk_ntets[cc] : 0;
+#pragma unroll
+        for (int h = 0; h < RR; h++) {
+            const int slot = h * SPR + rs;
+            g_t[h] = slot < g_nt ? pat.chunk_tets[cc * CHUNK_TMAX + slot] : 0;
+        }
+    };
+    auto prefetch = [&](int64_t cc) {
+        f_q0 = 0; f_ninc = 0; f_nr = 0; f_perm = 0;
+        f_nt = g_nt;
+#pragma unroll
+        for (int h = 0; h < 2; h++) { f_pp[h] = 0; f_row[h] = 0; }
+#pragma unroll
+        for (int h = 0; h < RI; h++) f_inc[h] = 0;
+#pragma unroll
+        for (int h = 0; h < RR; h++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) f_rec[h][q] = make_double2(0.0, 0.0);
+        if (cc >= pat.nchunk) return;
+        const int64_t kk = cc * CH;
+        const int npp = (int)(pat.nnz_ent - kk < CH ? pat.nnz_ent - kk : CH);
+        f_q0 = pat.pair_ptr[kk];
+        f_ninc = pat.pair_ptr[kk + npp] - f_q0;
+        const int32_t r0 = pat.chunk_row0[cc];
+        f_nr = pat.chunk_row0[cc + 1] - r0 + 2;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int i = tid + h * CH;
+            if (i <= npp) f_pp[h] = pat.pair_ptr[kk + i];
+            if (i < f_nr) f_row[h] = pat.ent_ptr[r0 + i];
+        }
+        const int64_t w0 = f_q0 >> 1;
+        const int nwords = (int)(((int64_t)f_q0 + f_ninc + 1) / 2 - w0);
+#pragma unroll
+        for (int h = 0; h < RI; h++) {
+            const int i = tid + h * CH;
+            if (i < nwords) f_inc[h] = reinterpret_cast<const uint32_t *>(pat.inc_local)[w0 + i];
+        }
+        f_perm = pat.chunk_perm[kk + tid];
+#pragma unroll
+        for (int h = 0; h < RR; h++)
+            if (h * SPR + rs < f_nt) {
+                const double2 *r = reinterpret_cast<const double2 *>(rec + (int64_t)g_t[h] * REC + rsub * 6);
+#pragma unroll
+                for (int q = 0; q < 3; q++) f_rec[h][q] = r[q];
+            }
+    };
+    tet_ids(blockIdx.x);
+    prefetch(blockIdx.x);
+    for (int64_t c = blockIdx.x; c < pat.nchunk; c += gridDim.x) {
+        __syncthreads();   // the previous chunk's staged output has been written
+        const int64_t k0 = c * CH;
+        const int np = (int)(pat.nnz_ent - k0 < CH ? pat.nnz_ent - k0 : CH);
+        const int32_t ioff = f_q0 & 1, nr = f_nr, nT = f_nt;
+#pragma unroll
+        for (int h = 0; h < RR; h++) {
+            const int slot = h * SPR + rs;
+            if (slot < nT) {
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    const int f = rsub * 6 + q;
+                    if (f < TD_FIELDS) td[f * TSP + slot] = (q & 1) ? f_rec[h][q >> 1].y : f_rec[h][q >> 1].x;
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int i = tid + h * CH;
+            if (i <= np) s_pp[i] = f_pp[h];
+            if (i < nr) s_row[i] = f_row[h];
+        }
+        {
+            const int nwords = (ioff + f_ninc + 1) / 2;
+#pragma unroll
+            for (int h = 0; h < RI; h++) {
+                const int i = tid + h * CH;
+                if (i < nwords) s_inc[i] = f_inc[h];
+            }
+        }
+        s_perm[tid] = (uint8_t)f_perm;
+        const int32_t qb = f_q0 - ioff;   // incidence index of s_inc's first 16-bit half
+        __syncthreads();
+        tet_ids(c + gridDim.x);
+        const int hw = (int)((c + NW / 2) % NW);   // wave hw takes the first (heaviest) 64 lane-order positions
+        const int lk = s_perm[(((tid >> 6) - hw + NW) % NW) * 64 + (tid & 63)];
+        const bool act = lk < np;
+        cplx aD[2][2], aF[2][2];
+        int32_t p0 = 0, deg = 0;
+        if (act) {
+            const int64_t k = k0 + lk;
+            int lo = 0, hi = nr - 2;   // the row of k: last i with s_row[i] <= k
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_row[mid] <= k) lo = mid; else hi = mid - 1;
+            }
+            p0 = s_row[lo];
+            deg = s_row[lo + 1] - p0;
+            const int q1 = s_pp[lk + 1] - qb;
+            const uint16_t *inc16 = reinterpret_cast<const uint16_t *>(s_inc);
+#pragma unroll 1
+            for (int q = s_pp[lk] - qb; q < q1; q++) fused_incidence<TSP>(rcp, td, inc16[q], exact_face_mass, aD, aF);
+        }
+        __syncthreads();   // Gram data no longer read: td becomes the output stage
+        prefetch(c + gridDim.x);
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) stg[m * CH + lk] = aD[m >> 1][m & 1];
+            s_out[lk] = make_int2(p0, deg);
+        }
+        __syncthreads();
+        int2 o = make_int2(0, 0);
+        if (tid < np) {
+            o = s_out[tid];
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                E[(int64_t)(m >> 1) * 2 * pat.nnz_ent + o.x + (int64_t)(m & 1) * o.y + k0 + tid] = stg[m * CH + tid];
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) stg[m * CH + lk] = aF[m >> 1][m & 1];
+        }
+        __syncthreads();
+        if (tid < np) {
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                B[(int64_t)(m >> 1) * 2 * pat.nnz_ent + o.x + (int64_t)(m & 1) * o.y + k0 + tid] = stg[m * CH + tid];
+        }
+    }
+}
+
+template <int CH>
+static void launch_staged(const emg_pattern *pat, unsigned grid, hipStream_t st, const double *rec, int efm,
+                          cplx *oE, cplx *oB) {
+    const int tm = pat->chunk_tmax;
+    if (tm <= 32)
+        k_assemble_staged<32, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB);
+    else if (tm <= 48)
+        k_assemble_staged<48, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB);
+    else if (tm <= 64)
+        k_assemble_staged<64, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB);
+    else
+        k_assemble_staged<CHUNK_TMAX, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB);
 }
 
 static int g_num_cus = 0;
@@ -997,19 +1216,14 @@ extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, c
             EMG_HIP(hipGetDevice(&dev));
             EMG_HIP(hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev));
         }
+        // 4 resident workgroups per CU (register bound: 4 waves per SIMD), two rounds of them
         const int64_t grid = std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * 8);
         const int efm = (flags & EMG_EXACT_FACE_MASS) ? 1 : 0;
         const cplx *ce = reinterpret_cast<const cplx *>(d_eps), *cn = reinterpret_cast<const cplx *>(d_nu);
         cplx *oE = reinterpret_cast<cplx *>(d_E), *oB = reinterpret_cast<cplx *>(d_B);
-        const int tm = pat->chunk_tmax;
-        if (tm <= 32)
-            k_assemble_fused<32><<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, ce, cn, efm, oE, oB);
-        else if (tm <= 48)
-            k_assemble_fused<48><<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, ce, cn, efm, oE, oB);
-        else if (tm <= 64)
-            k_assemble_fused<64><<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, ce, cn, efm, oE, oB);
-        else
-            k_assemble_fused<CHUNK_TMAX><<<(unsigned)grid, 256, 0, st>>>(*pat, d_nodes, d_tets, ce, cn, efm, oE, oB);
+        if (!mp->tet_rec) EMG_HIP(hipMalloc(&mp->tet_rec, (size_t)pat->nt * REC * sizeof(double)));
+        k_tet_gram<<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec);
+        launch_staged<CHUNK>(pat, (unsigned)grid, st, mp->tet_rec, efm, oE, oB);
         EMG_LAUNCH_CHECK();
         return 0;
     }
