@@ -24,11 +24,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sweep freq-pts/s at ≤1e-8 FP64 residual, learned vs plain GPU; assembly Mtet/s"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_assemble_fused launch on the 1,005,000-tet workload, from
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this benchmark
-# (--sweep none --copy-calibration): FETCH 881,966 KiB x 2 + WRITE 8,791,550 KiB x 1,
-# the factors a 2 GiB device-to-device copy in the same runs gives.
-ASSEMBLY_TRAFFIC_BYTES = (881_966 * 2 + 8_791_550) * 1024
+# HBM bytes per numeric assembly (k_tet_gram + k_assemble_staged) on the
+# 1,005,000-tet workload, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
+# runs of this benchmark (--sweep none --copy-calibration): FETCH (52,925 +
+# 1,702,889) KiB x 2 + WRITE (425,980 + 8,794,723) KiB x 1, the factors a 2 GiB
+# device-to-device copy in the same runs gives.
+ASSEMBLY_TRAFFIC_BYTES = ((52_925 + 1_702_889) * 2 + 425_980 + 8_794_723) * 1024
 FP64_PEAK_TFLOPS = 78.6
 # FP64 flop convention per tet for the dense contraction of the closed form
 FLOP_PER_TET = 15449
@@ -503,7 +504,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS,
                      "traffic": ASSEMBLY_TRAFFIC_BYTES if not (args.two_phase or args.exact_face_mass) else None,
-                     "kernel": "k_tet_blocks + k_gather_iso" if args.two_phase else "k_assemble_fused", "kernel_ms": kern_ms, "alg_bytes": alg_bytes,
+                     "kernel": "k_tet_blocks + k_gather_iso" if args.two_phase else "k_tet_gram + k_assemble_staged",
+                     "kernel_ms": kern_ms, "alg_bytes": alg_bytes,
                      "fp64_tflops": flops / (kern_ms * 1e-3) / 1e12,
                      "fp64_frac": flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
     }

@@ -981,8 +981,9 @@ __device__ __forceinline__ void fused_incidence(const uint32_t *rcp, const doubl
     }
 }
 
-// Gram record of every tet (REC doubles each, fields as fused_tet_gram): the
-// first step of the numeric assembly, so each tet's geometry is computed once.
+// Gram record of every tet (REC doubles each, fields as fused_tet_gram, the
+// rest padding): the first step of the numeric assembly, so each tet's geometry
+// is computed once.  48 doubles = 384 bytes = three whole 128-byte lines.
 static constexpr int REC = 48;
 __global__ void __launch_bounds__(256) k_tet_gram(int64_t nt, const double *__restrict__ nodes,
                                                   const int32_t *__restrict__ tets, const cplx *__restrict__ eps,
@@ -1023,9 +1024,9 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
     constexpr int NID = NINC / 2 + 1;       // incidence codes as 32-bit words (first one may be half used)
     constexpr int RI = (NID + CH - 1) / CH; // words per lane
     constexpr int NW = CH / 64;
-    constexpr int SPR = CH / 8;             // tet records per round: 8 lanes x 6 doubles each
+    constexpr int SPR = CH / 8;             // tet records per round: 8 lanes per record, 7 of them load 6 doubles
     constexpr int RR = (TS + SPR - 1) / SPR;
-    static_assert(8 * 6 >= TD_FIELDS && 6 * 8 <= REC, "8 lanes of 6 doubles cover a record");
+    static_assert(7 * 6 >= TD_FIELDS && 7 * 6 <= REC, "7 lanes of 6 doubles cover a record");
     __shared__ uint32_t rcp[100 * RECIPE_BYTES / 4];
     __shared__ __align__(16) double td[TD > SD ? TD : SD];
     __shared__ int32_t s_pp[CH + 1];        // entry -> first incidence (absolute)
@@ -1086,7 +1087,7 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
         f_perm = pat.chunk_perm[kk + tid];
 #pragma unroll
         for (int h = 0; h < RR; h++)
-            if (h * SPR + rs < f_nt) {
+            if (h * SPR + rs < f_nt && rsub < 7) {
                 const double2 *r = reinterpret_cast<const double2 *>(rec + (int64_t)g_t[h] * REC + rsub * 6);
 #pragma unroll
                 for (int q = 0; q < 3; q++) f_rec[h][q] = r[q];
@@ -1102,7 +1103,7 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
 #pragma unroll
         for (int h = 0; h < RR; h++) {
             const int slot = h * SPR + rs;
-            if (slot < nT) {
+            if (slot < nT && rsub < 7) {
 #pragma unroll
                 for (int q = 0; q < 6; q++) {
                     const int f = rsub * 6 + q;
