@@ -81,8 +81,10 @@ static constexpr int SUB = 16;  // lanes per CSR row
 // rows fetched a third fewer bytes but ran 30 % slower.)
 
 // y = mask.(A (mask.x)) + (1-mask).x   (mask == nullptr: plain y = A x)
-// If `b` is given, y = b - (that).  L lanes share one CSR row.
-template <typename R, int L = SUB>
+// If `b` is given, y = b - (that).  L lanes share one CSR row.  MC: apply the
+// column mask per entry (false when the masked columns are already zero in
+// vals, as in the solver's complex64 copy).
+template <typename R, int L, bool MC>
 __global__ void __launch_bounds__(256) k_spmv(int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
                                               const R *__restrict__ vals, const uint8_t *__restrict__ mask,
                                               const R *__restrict__ x, const R *__restrict__ b, R *__restrict__ y) {
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(256) k_spmv(int64_t n, const int32_t *__restri
         const int32_t p0 = rowptr[row], p1 = rowptr[row + 1];
         for (int32_t p = p0 + lane; p < p1; p += L) {
             const int32_t c = cols[p];
-            if (mask != nullptr && !mask[c]) continue;
+            if (MC && mask != nullptr && !mask[c]) continue;
             acc = cadd(acc, cmul(ld(vals, p), ld(x, c)));
         }
     }
@@ -125,17 +127,27 @@ static int spmv_lanes() {
     return lanes;
 }
 
-template <typename R>
-static void launch_spmv(hipStream_t st, int64_t n, const int32_t *rowptr, const int32_t *cols, const R *vals,
-                        const uint8_t *mask, const R *x, const R *b, R *y) {
+template <typename R, bool MC>
+static void launch_spmv_l(hipStream_t st, int64_t n, const int32_t *rowptr, const int32_t *cols, const R *vals,
+                          const uint8_t *mask, const R *x, const R *b, R *y) {
     const int L = spmv_lanes();
     const unsigned grid = (unsigned)((n * L + 255) / 256);
     switch (L) {
-        case 4: k_spmv<R, 4><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
-        case 8: k_spmv<R, 8><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
-        case 32: k_spmv<R, 32><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
-        default: k_spmv<R, 16><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
+        case 4: k_spmv<R, 4, MC><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
+        case 8: k_spmv<R, 8, MC><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
+        case 32: k_spmv<R, 32, MC><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
+        default: k_spmv<R, 16, MC><<<grid, 256, 0, st>>>(n, rowptr, cols, vals, mask, x, b, y); break;
     }
+}
+
+// mask_cols false: vals already holds zeros in the masked columns.
+template <typename R>
+static void launch_spmv(hipStream_t st, int64_t n, const int32_t *rowptr, const int32_t *cols, const R *vals,
+                        const uint8_t *mask, const R *x, const R *b, R *y, bool mask_cols = true) {
+    if (mask_cols)
+        launch_spmv_l<R, true>(st, n, rowptr, cols, vals, mask, x, b, y);
+    else
+        launch_spmv_l<R, false>(st, n, rowptr, cols, vals, mask, x, b, y);
 }
 
 // ---------------------------------------------------------------- BLAS-1
@@ -475,6 +487,16 @@ __global__ void __launch_bounds__(RB) k_bj_setup(int64_t n_ent, const int32_t *_
 __global__ void __launch_bounds__(RB) k_to_c64(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
     for (int64_t e = blockIdx.x * (int64_t)RB + threadIdx.x; e < 2 * n; e += (int64_t)gridDim.x * RB) b[e] = (float)a[e];
 }
+// complex64 copy of the matrix values with the masked (PEC) columns set to zero
+__global__ void __launch_bounds__(RB) k_to_c64_masked(int64_t nnz, const int32_t *__restrict__ cols,
+                                                      const uint8_t *__restrict__ mask, const double *__restrict__ a,
+                                                      float *__restrict__ b) {
+    for (int64_t p = blockIdx.x * (int64_t)RB + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * RB) {
+        const bool keep = mask == nullptr || mask[cols[p]];
+        b[2 * p] = keep ? (float)a[2 * p] : 0.0f;
+        b[2 * p + 1] = keep ? (float)a[2 * p + 1] : 0.0f;
+    }
+}
 __global__ void __launch_bounds__(RB) k_add_c64_to_c128(int64_t n, const float *__restrict__ d, double *__restrict__ x) {
     for (int64_t e = blockIdx.x * (int64_t)RB + threadIdx.x; e < 2 * n; e += (int64_t)gridDim.x * RB) x[e] += (double)d[e];
 }
@@ -678,7 +700,7 @@ extern "C" int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const 
     s->vals = d_vals;
     s->mask = d_mask;
     k_bj_setup<<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, s->rowptr, s->cols, d_vals, d_mask, s->inv64, s->inv32);
-    if (s->precision == 32) k_to_c64<<<1024, RB, 0, st>>>(s->nnz, d_vals, s->vals32);
+    if (s->precision == 32) k_to_c64_masked<<<2048, RB, 0, st>>>(s->nnz, s->cols, d_mask, d_vals, s->vals32);
     if (s->np2 > 0) {
         EMG_HIP(hipMemsetAsync(s->dS, 0, s->np2 * 16, st));
         k_gdiag<<<emg::grid_for(s->n, RB), RB, 0, st>>>(s->n, s->rowptr, s->cols, d_vals, d_mask, s->g_rowptr,
@@ -872,7 +894,8 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
     const int nbm = mdot_blocks(n, nv);
     const dim3 grid(nbm, (nv + DT - 1) / DT);
     apply_prec<R>(s, inv, vj, z, st);
-    launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn);
+    // the complex64 copy has its masked columns zeroed (k_to_c64_masked)
+    launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn, sizeof(R) == 8);
     k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, nullptr);
     k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, h1, 0, nullptr);
     k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, h1, vn, s->pn, nullptr);
@@ -945,7 +968,7 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     double resid = 1.0;
     while (true) {
         // r = rhs - K u  ->  v0 = r / |r|
-        launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, u, rhs, w);
+        launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, u, rhs, w, sizeof(R) == 8);
         k_norm2<R><<<nb, RB, 0, st>>>(n, w, s->partial);
         k_scale_by_norm<R><<<nb, RB, 0, st>>>(n, s->partial, nb, w, V, s->nrm);
         double beta;
