@@ -120,6 +120,7 @@ def bench_sweep(args, dev):
         phys.set_frequency(fl)
         st = phys.solveroutine
         st.precision, st.tol, st.store_fields, st.sweep, st.verbose = prec, args.sweep_tol, False, mode, True
+        st.poly_degree = args.poly_degree if mode == "basis" else 1
         phys.problem()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -135,6 +136,7 @@ def bench_sweep(args, dev):
                     "all_converged": bool(all(r.converged and r.residual <= args.sweep_tol for r in recs))}
         if mode == "basis":
             out[key]["basis_size"] = int(data.info.get("basis_size", 0))
+            out[key]["poly_degree"] = args.poly_degree
         print(json.dumps({key: out[key]}), file=sys.stderr, flush=True)
     if args.learned and "learned" in runs_wanted:
         from emerge_amd.fdsolve import GPUSolver
@@ -160,7 +162,7 @@ def bench_sweep(args, dev):
         phys.set_frequency(freqs)
         st = phys.solveroutine
         st.precision, st.tol, st.store_fields, st.sweep, st.learned = 32, args.sweep_tol, False, "basis", net
-        st.verbose, st.maxiter = True, 20000
+        st.verbose, st.maxiter, st.poly_degree = True, 20000, args.poly_degree
         t0 = time.perf_counter()
         data = phys.frequency_domain()
         torch.cuda.synchronize()
@@ -405,6 +407,8 @@ def main():
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip training / using the learned smoother in the sweep")
     ap.add_argument("--learned-steps", type=int, default=60)
+    ap.add_argument("--poly-degree", type=int, default=6,
+                    help="degree of the GMRES polynomial around the smoother in the basis sweeps (1 = off)")
     ap.add_argument("--sweep-runs", default="basis,plain,learned",
                     help="comma-separated sweep variants to run: basis, plain, learned")
     ap.add_argument("--eigen", action="store_true", help="also run the cavity eigenmode workload")

@@ -42,6 +42,8 @@ SIGNATURES: dict[str, tuple] = {
     "emg_interp_points": (ctypes.c_int, [I64, P, P, P, P, P, I64, ctypes.c_int, P, P, P]),
     "emg_solver_set_weights": (ctypes.c_int, [P, P, P]),
     "emg_solver_smoother_parts": (ctypes.c_int, [P, P, P, P, P]),
+    "emg_solver_set_poly": (ctypes.c_int, [P, ctypes.c_int, P]),
+    "emg_solver_arnoldi_probe": (ctypes.c_int, [P, ctypes.c_int, P, P, P]),
     "emg_csr_real_apply": (ctypes.c_int, [I64, P, P, P, P, P, ctypes.c_int, P]),
     "emg_gnn_gemm": (ctypes.c_int, [I64, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_float, P, P]),
     "emg_gnn_gemm_nt": (ctypes.c_int, [I64, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_float, P, P]),

@@ -574,19 +574,27 @@ struct emg_solver {
     // optional learned weights of the smoother (caller-owned complex128):
     // w_e per entity scales its 2x2 block inverse, w_p per P2 node its diagonal
     const double *w_e = nullptr, *w_p = nullptr;
+    // optional polynomial acceleration of the smoother (emg_solver_set_poly):
+    // M p(K M) with 1 - z p(z) = prod_i (1 - z c_i), c_i = 1/theta_i on the device
+    int poly_deg = 1;
+    double *poly_c = nullptr;                               // POLY_MAX complex coefficients
+    double *pp = nullptr, *pt = nullptr, *py = nullptr, *pu = nullptr;   // work vectors (n complex each)
     struct graphs {
         std::vector<hipGraphExec_t> exec;
         const void *vals = nullptr, *inv = nullptr, *mask = nullptr;
         int64_t np2 = 0;
+        int poly = 1;
     } gs[2];                      // [0] complex128 steps, [1] complex64 steps
 };
+static constexpr int POLY_MAX = 16;
 
 static void free_solver(emg_solver *s) {
     if (!s) return;
     for (auto &g : s->gs)
         for (hipGraphExec_t e : g.exec) (void)hipGraphExecDestroy(e);
     void *ps[] = {s->vals32, s->inv64, s->inv32, s->V, s->w, s->z, s->r, s->d, s->partial, s->h, s->nrm,
-                  s->arn, s->pn, s->flag, s->g_vals32, s->gt_vals32, s->dS, s->dsinv64, s->tg, s->dsinv32};
+                  s->arn, s->pn, s->flag, s->g_vals32, s->gt_vals32, s->dS, s->dsinv64, s->tg, s->dsinv32,
+                  s->poly_c, s->pp, s->pt, s->py, s->pu};
     for (void *p : ps) (void)hipFree(p);
     if (s->res_host) (void)hipHostFree(s->res_host);
     if (s->cap) (void)hipStreamDestroy(s->cap);
@@ -608,6 +616,54 @@ static void apply_prec(emg_solver *s, const R *inv, const R *x, R *y, hipStream_
     }
 }
 
+
+// y = c[0] x   (c: complex128 coefficients on the device)
+template <typename R>
+__global__ void __launch_bounds__(RB) k_poly_first(int64_t n, const double *__restrict__ c, const R *__restrict__ x,
+                                                   R *__restrict__ y) {
+    const double cr = c[0], ci = c[1];
+    for (int64_t e = blockIdx.x * (int64_t)RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) {
+        const cx<R> v = ld(x, e);
+        st(y, e, cx<R>((R)(cr * v.re - ci * v.im), (R)(cr * v.im + ci * v.re)));
+    }
+}
+// p = q - c[i] t ;  y += c[i+1] p   (q may be p)
+template <typename R>
+__global__ void __launch_bounds__(RB) k_poly_step(int64_t n, const double *__restrict__ c, int i, const R *q,
+                                                  const R *__restrict__ t, R *p, R *__restrict__ y) {
+    const double ar = c[2 * i], ai = c[2 * i + 1], br = c[2 * i + 2], bi = c[2 * i + 3];
+    for (int64_t e = blockIdx.x * (int64_t)RB + threadIdx.x; e < n; e += (int64_t)gridDim.x * RB) {
+        const cx<R> qv = ld(q, e), tv = ld(t, e), yv = ld(y, e);
+        const double pr = qv.re - (ar * tv.re - ai * tv.im), pi = qv.im - (ar * tv.im + ai * tv.re);
+        st(p, e, cx<R>((R)pr, (R)pi));
+        st(y, e, cx<R>((R)(yv.re + br * pr - bi * pi), (R)(yv.im + br * pi + bi * pr)));
+    }
+}
+
+// The preconditioner: the smoother M, or with a polynomial of degree d > 1,
+// M p(K M) x evaluated as  y = c_0 x, q_0 = x;  q_{i+1} = q_i - c_i K M q_i,
+// y += c_{i+1} q_{i+1}  (i < d-1);  out = M y.  d - 1 extra SpMVs and d - 1
+// extra smoother applications per call.
+template <typename R>
+static void apply_prec_poly(emg_solver *s, const R *vals, const R *inv, const R *x, R *out, hipStream_t st) {
+    if (s->poly_deg <= 1) {
+        apply_prec<R>(s, inv, x, out, st);
+        return;
+    }
+    const int64_t n = s->n;
+    const unsigned g = (unsigned)std::min<int64_t>((n + RB - 1) / RB, 2048);
+    R *pp = reinterpret_cast<R *>(s->pp), *pt = reinterpret_cast<R *>(s->pt);
+    R *py = reinterpret_cast<R *>(s->py), *pu = reinterpret_cast<R *>(s->pu);
+    k_poly_first<R><<<g, RB, 0, st>>>(n, s->poly_c, x, py);
+    const R *q = x;
+    for (int i = 0; i < s->poly_deg - 1; i++) {
+        apply_prec<R>(s, inv, q, pu, st);
+        launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, pu, nullptr, pt, sizeof(R) == 8);
+        k_poly_step<R><<<g, RB, 0, st>>>(n, s->poly_c, i, q, pt, pp, py);
+        q = pp;
+    }
+    apply_prec<R>(s, inv, py, out, st);
+}
 
 // Blocks per tile of the multi-dot: about 2048 blocks in total, so each block
 // streams many rows and the per-block reduction is amortised.
@@ -658,6 +714,8 @@ extern "C" int emg_solver_create(int64_t n, int64_t nnz, const int32_t *d_rowptr
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->res_host, (restart + 8) * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking);
     if (precision == 32) { A((void **)&s->inv32, s->n_ent * 4 * 8); A((void **)&s->vals32, nnz * 8); }
+    A((void **)&s->poly_c, 2 * POLY_MAX * sizeof(double));
+    A((void **)&s->pp, n * 16); A((void **)&s->pt, n * 16); A((void **)&s->py, n * 16); A((void **)&s->pu, n * 16);
     if (e != hipSuccess) {
         emg::set_error(std::string("emg_solver_create: ") + hipGetErrorString(e));
         free_solver(s);
@@ -733,6 +791,80 @@ extern "C" int emg_solver_smoother_parts(emg_solver *s, const double *d_r, doubl
         k_gt_apply<double><<<(unsigned)((s->np2 * SUB + 255) / 256), 256, 0, st>>>(s->np2, s->gt_rowptr, s->gt_cols,
                                                                                   s->gt_vals, s->dsinv64, d_r, d_t);
     EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+// Polynomial of the preconditioner: degree d (1 = none) and its coefficients
+// c_i = 1/theta_i (complex128, host array of 2 d doubles) in application order.
+extern "C" int emg_solver_set_poly(emg_solver *s, int degree, const double *h_coef) {
+    EMG_CHECK(s != nullptr, "emg_solver_set_poly: null solver");
+    EMG_CHECK(degree >= 1 && degree <= POLY_MAX, "emg_solver_set_poly: degree must be in [1, 16]");
+    EMG_CHECK(degree == 1 || h_coef != nullptr, "emg_solver_set_poly: null coefficients");
+    if (degree > 1) {
+        for (int i = 0; i < 2 * degree; i++)
+            EMG_CHECK(std::isfinite(h_coef[i]), "emg_solver_set_poly: non-finite coefficient");
+        EMG_HIP(hipMemcpy(s->poly_c, h_coef, 2 * degree * sizeof(double), hipMemcpyHostToDevice));
+    }
+    s->poly_deg = degree;
+    return 0;
+}
+
+__global__ void __launch_bounds__(RB) k_scale_c128(int64_t n, double a, double *__restrict__ x) {
+    for (int64_t e = blockIdx.x * (int64_t)RB + threadIdx.x; e < 2 * n; e += (int64_t)gridDim.x * RB) x[e] *= a;
+}
+
+// d steps of Arnoldi (complex128, two classical Gram-Schmidt passes) on K M
+// (M = the smoother without polynomial, K masked) from d_v0; writes the
+// (d + 1) x d Hessenberg matrix column-major as complex128 to h_out (host).
+// Uses the solver's Krylov basis storage, so it must not run during a solve.
+extern "C" int emg_solver_arnoldi_probe(emg_solver *s, int d, const double *d_v0, double *h_out, void *stream) {
+    EMG_CHECK(s && d_v0 && h_out, "emg_solver_arnoldi_probe: null argument");
+    EMG_CHECK(s->vals != nullptr, "emg_solver_arnoldi_probe: emg_solver_set_matrix was not called");
+    const int64_t n = s->n;
+    const size_t es = s->precision == 64 ? 16 : 8;
+    EMG_CHECK(d >= 1 && d <= POLY_MAX && (size_t)(d + 1) * 16 <= (size_t)(s->restart + 1) * es,
+              "emg_solver_arnoldi_probe: d out of range for this solver");
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = s->nblk;
+    double *V = s->V;   // d + 1 complex128 vectors
+    double *z = s->z;
+    std::vector<double> H(2 * (size_t)(d + 1) * d, 0.0), pn(nb), hh(2 * (d + 1));
+    auto host_norm = [&](const double *partial) -> double {
+        EMG_HIP(hipMemcpyAsync(pn.data(), partial, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+        EMG_HIP(hipStreamSynchronize(st));
+        double t = 0;
+        for (double v : pn) t += v;
+        return std::sqrt(t);
+    };
+    EMG_HIP(hipMemcpyAsync(V, d_v0, n * 16, hipMemcpyDeviceToDevice, st));
+    k_norm2<double><<<nb, RB, 0, st>>>(n, V, s->partial);
+    const double b0 = host_norm(s->partial);
+    EMG_CHECK(b0 > 0 && std::isfinite(b0), "emg_solver_arnoldi_probe: zero or non-finite start vector");
+    k_scale_c128<<<nb, RB, 0, st>>>(n, 1.0 / b0, V);
+    for (int j = 0; j < d; j++) {
+        double *vj = V + (size_t)j * n * 2, *w = V + (size_t)(j + 1) * n * 2;
+        const int nv = j + 1, nbm = mdot_blocks(n, nv);
+        const dim3 grid(nbm, (nv + DT - 1) / DT);
+        apply_prec<double>(s, s->inv64, vj, z, st);
+        launch_spmv<double>(st, n, s->rowptr, s->cols, s->vals, s->mask, z, nullptr, w);
+        double wn = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            k_mdot<double><<<grid, RB, 0, st>>>(n, nv, V, n, w, s->partial, nullptr);
+            k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, s->h, 0, nullptr);
+            k_maxpy_norm<double><<<nb, RB, 0, st>>>(n, nv, V, n, s->h, w, s->pn, nullptr);
+            EMG_HIP(hipMemcpyAsync(hh.data(), s->h, 2 * nv * sizeof(double), hipMemcpyDeviceToHost, st));
+            wn = host_norm(s->pn);
+            for (int i = 0; i < nv; i++) {
+                H[2 * ((size_t)j * (d + 1) + i)] += hh[2 * i];
+                H[2 * ((size_t)j * (d + 1) + i) + 1] += hh[2 * i + 1];
+            }
+        }
+        H[2 * ((size_t)j * (d + 1) + j + 1)] = wn;
+        EMG_CHECK(wn > 0 && std::isfinite(wn), "emg_solver_arnoldi_probe: breakdown");
+        k_scale_c128<<<nb, RB, 0, st>>>(n, 1.0 / wn, w);
+    }
+    EMG_HIP(hipStreamSynchronize(st));
+    memcpy(h_out, H.data(), H.size() * sizeof(double));
     return 0;
 }
 
@@ -893,7 +1025,7 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
     const arn_off o(m);
     const int nbm = mdot_blocks(n, nv);
     const dim3 grid(nbm, (nv + DT - 1) / DT);
-    apply_prec<R>(s, inv, vj, z, st);
+    apply_prec_poly<R>(s, vals, inv, vj, z, st);
     // the complex64 copy has its masked columns zeroed (k_to_c64_masked)
     launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn, sizeof(R) == 8);
     k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, nullptr);
@@ -912,7 +1044,9 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
 template <typename R>
 static int ensure_graphs(emg_solver *s, const R *vals, const R *inv) {
     auto &g = s->gs[sizeof(R) == 8 ? 0 : 1];
-    if (!g.exec.empty() && g.vals == vals && g.inv == inv && g.mask == s->mask && g.np2 == s->np2) return 0;
+    if (!g.exec.empty() && g.vals == vals && g.inv == inv && g.mask == s->mask && g.np2 == s->np2 &&
+        g.poly == s->poly_deg)
+        return 0;
     for (hipGraphExec_t e : g.exec) (void)hipGraphExecDestroy(e);
     g.exec.clear();
     const int m = s->restart;
@@ -931,6 +1065,7 @@ static int ensure_graphs(emg_solver *s, const R *vals, const R *inv) {
     g.inv = inv;
     g.mask = s->mask;
     g.np2 = s->np2;
+    g.poly = s->poly_deg;
     return 0;
 }
 
@@ -944,7 +1079,7 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     R *V = reinterpret_cast<R *>(s->V);
     R *w = reinterpret_cast<R *>(s->w);
     R *z = reinterpret_cast<R *>(s->z);
-    auto prec = [&](const R *x, R *y) { apply_prec<R>(s, inv, x, y, st); };
+    auto prec = [&](const R *x, R *y) { apply_prec_poly<R>(s, vals, inv, x, y, st); };
     double bnorm = 0;
     {
         k_norm2<R><<<nb, RB, 0, st>>>(n, rhs, s->partial);
@@ -1069,7 +1204,12 @@ extern "C" int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, d
             const double rs = 1.0 / (rel * bn);
             k_c128_to_c64_scaled<<<1024, RB, 0, st>>>(n, s->r, rs, r32);
             EMG_HIP(hipMemsetAsync(d32, 0, n * 8, st));
-            const double inner_tol = std::max(0.5 * tol / rel, 1e-7);
+            static double floor_tol = -1.0;
+            if (floor_tol < 0) {
+                const char *e = getenv("EMG_INNER_FLOOR");
+                floor_tol = e ? atof(e) : 1e-7;
+            }
+            const double inner_tol = std::max(0.5 * tol / rel, floor_tol);
             int it;
             double est;
             rc = gmres_core<float>(s, s->vals32, s->inv32, r32, d32, inner_tol,

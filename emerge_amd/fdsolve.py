@@ -196,17 +196,55 @@ class FrequencyProblem:
         return self._gradient
 
 
+def poly_coefficients(H: np.ndarray) -> np.ndarray | None:
+    """Coefficients c_i = 1/theta_i of the GMRES polynomial preconditioner from
+    the (d+1) x d Hessenberg matrix of d Arnoldi steps on K M: theta are the
+    harmonic Ritz values (eigenvalues of H_d + |h_{d+1,d}|^2 H_d^{-H} e_d e_d^T,
+    the roots of the GMRES residual polynomial of degree d), in Leja order
+    (largest first, then each maximising the product of distances to those
+    before) so the product form 1 - z p(z) = prod (1 - z/theta_i) is evaluated
+    stably.  None if a root is zero or not finite."""
+    H = np.asarray(H, dtype=np.complex128)
+    d = H.shape[1]
+    if H.shape[0] != d + 1 or d < 1:
+        raise ValueError(f"H must be (d+1) x d, got {H.shape}")
+    Hd = H[:d, :d]
+    ed = np.zeros(d)
+    ed[-1] = 1.0
+    try:
+        f = np.linalg.solve(Hd.conj().T, ed)
+        theta = np.linalg.eigvals(Hd + abs(H[d, d - 1]) ** 2 * np.outer(f, ed))
+    except np.linalg.LinAlgError:
+        return None
+    if not np.all(np.isfinite(theta)) or np.min(np.abs(theta)) == 0.0:
+        return None
+    order = [int(np.argmax(np.abs(theta)))]
+    while len(order) < d:
+        rest = [i for i in range(d) if i not in order]
+        score = [np.sum(np.log(np.abs(theta[i] - theta[order]) + 1e-300)) for i in rest]
+        order.append(rest[int(np.argmax(score))])
+    return 1.0 / theta[order]
+
+
 class GPUSolver:
     """Restarted GMRES on the GPU (emg_solver_* in the C ABI).
 
     preconditioner: "hybrid" (2x2 block Jacobi plus the gradient correction
     G diag(G^T K G)^{-1} G^T, emerge_amd.gradient) or "jacobi" (block Jacobi only).
+    poly_degree: d > 1 wraps the smoother M in the GMRES polynomial of degree d,
+    M p(K M), recomputed at every set_matrix from d Arnoldi steps on K M
+    (poly_coefficients); each GMRES step then applies K and M d times but the
+    Krylov space, and with it the orthogonalisation work, grows d times slower.
     """
 
     def __init__(self, problem: FrequencyProblem, restart: int = 150, precision: int = 64,
-                 preconditioner: str = "hybrid"):
+                 preconditioner: str = "hybrid", poly_degree: int = 1):
         if preconditioner not in ("hybrid", "jacobi"):
             raise ValueError(f"preconditioner must be 'hybrid' or 'jacobi', not {preconditioner!r}")
+        if not 1 <= int(poly_degree) <= 16:
+            raise ValueError(f"poly_degree must be in [1, 16], not {poly_degree!r}")
+        self.poly_degree = int(poly_degree)
+        self.poly_active = 1
         self.L = _lib.load()
         self.p = problem
         h = ctypes.c_void_p()
@@ -233,6 +271,7 @@ class GPUSolver:
             self.h = None
 
     learned = None          # LearnedSmoother whose weights scale the smoother at each frequency
+    POLY_STEP_CAP = 3000    # GMRES steps allowed with the polynomial before the plain smoother takes over
 
     def set_weights(self, we, wp):
         """Learned smoother weights (complex128 device tensors or None) for the next set_matrix."""
@@ -245,25 +284,63 @@ class GPUSolver:
         self._K = K
         if self.learned is not None and freq is not None:
             self.set_weights(*self.learned.weights(freq))
-        _lib.check(self.L.emg_solver_set_matrix(self.h, _lib.ptr(K), _lib.ptr(self.p.mask), _lib.stream_ptr()),
+        self._set_matrix_c()
+
+    def _set_matrix_c(self):
+        _lib.check(self.L.emg_solver_set_matrix(self.h, _lib.ptr(self._K), _lib.ptr(self.p.mask), _lib.stream_ptr()),
                    "emg_solver_set_matrix")
+        if self.poly_degree > 1:
+            self._update_poly()
+
+    def _update_poly(self):
+        """Polynomial of the preconditioner for the current matrix (plain smoother if it fails)."""
+        torch = self.p.torch
+        d = self.poly_degree
+        if getattr(self, "_v0", None) is None:
+            g = torch.Generator(device=self.p.dev)
+            g.manual_seed(20251)
+            v = torch.randn((self.p.N, 2), generator=g, dtype=torch.float64, device=self.p.dev)
+            self._v0 = torch.view_as_complex(v).contiguous() * self.p.mask.to(torch.float64)
+        _lib.check(self.L.emg_solver_set_poly(self.h, 1, None), "emg_solver_set_poly")
+        H = np.zeros((d, d + 1), dtype=np.complex128)     # column-major (d+1) x d
+        _lib.check(self.L.emg_solver_arnoldi_probe(self.h, d, _lib.ptr(self._v0), H.ctypes.data, _lib.stream_ptr()),
+                   "emg_solver_arnoldi_probe")
+        c = poly_coefficients(H.T)
+        if c is None:
+            self.poly_active = 1
+            return
+        cc = np.ascontiguousarray(c, dtype=np.complex128)
+        _lib.check(self.L.emg_solver_set_poly(self.h, d, cc.ctypes.data), "emg_solver_set_poly")
+        self.poly_active = d
 
     def solve(self, b, x0=None, tol: float = 1e-8, maxiter: int = 20000, max_refine: int = 0):
-        """GMRES from x0; with learned smoother weights active, a solve that does not converge is
-        continued from its last iterate with the plain smoother (info.reserved = 1 marks it)."""
+        """GMRES from x0.  With the polynomial active, a solve that has not converged after
+        POLY_STEP_CAP steps continues from its last iterate without it (info.reserved bit 2;
+        info.iterations then counts smoother applications).  With learned smoother weights
+        active, a solve that does not converge is continued from its last iterate with the
+        plain smoother (bit 1)."""
         torch = self.p.torch
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         info = _lib.SolveInfo()
-        _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, maxiter, max_refine,
+        cap = maxiter if self.poly_active <= 1 else min(maxiter, self.POLY_STEP_CAP)
+        _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, cap, max_refine,
                                            ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
+        if self.poly_active > 1 and not info.converged:
+            # the polynomial did not pay off for this matrix: continue with the plain smoother
+            used = info.iterations * self.poly_active
+            _lib.check(self.L.emg_solver_set_poly(self.h, 1, None), "emg_solver_set_poly")
+            self.poly_active = 1
+            _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, maxiter, max_refine,
+                                               ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
+            info.iterations += used
+            info.reserved |= 2
         w = getattr(self, "_w", (None, None))
         if not info.converged and (w[0] is not None or w[1] is not None):
             used = info.iterations
             self.set_weights(None, None)
-            _lib.check(self.L.emg_solver_set_matrix(self.h, _lib.ptr(self._K), _lib.ptr(self.p.mask),
-                                                    _lib.stream_ptr()), "emg_solver_set_matrix")
+            self._set_matrix_c()
             _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, maxiter, max_refine,
                                                ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
             info.iterations += used
-            info.reserved = 1
+            info.reserved |= 1
         return x, info

@@ -35,12 +35,14 @@ class GPUSolveSettings:
     basis_tol: GMRES solves that enlarge the basis go to tol * basis_tol
     verbose: one progress line per block of the basis sweep on stderr
     learned: a trained emerge_amd.learned.LearnedSmoother (None = plain hybrid smoother)
+    poly_degree: d > 1 wraps the smoother in the degree-d GMRES polynomial of K M,
+                 recomputed per frequency (emerge_amd.fdsolve.poly_coefficients); 1 = off
     """
 
     def __init__(self, precision: int = 32, tol: float = 1e-8, restart: int = 150, maxiter: int = 60000,
                  max_refine: int = 0, warm_start: bool = True, store_fields: bool = True, sweep: str = "basis",
                  max_basis: int = 128, preconditioner: str = "hybrid", basis_tol: float = 1e-1,
-                 verbose: bool = False, learned=None):
+                 verbose: bool = False, learned=None, poly_degree: int = 1):
         if sweep not in ("basis", "plain"):
             raise ValueError(f"sweep must be 'basis' or 'plain', not {sweep!r}")
         self.sweep = sweep
@@ -49,6 +51,7 @@ class GPUSolveSettings:
         self.basis_tol = basis_tol
         self.verbose = verbose
         self.learned = learned
+        self.poly_degree = poly_degree
         self.precision = precision
         self.tol = tol
         self.restart = restart
@@ -150,7 +153,8 @@ class Electrodynamics3D:
         st = self.solveroutine
         prob = self.problem()
         sampler = PortSampler(prob)
-        solver = GPUSolver(prob, restart=st.restart, precision=st.precision, preconditioner=st.preconditioner)
+        solver = GPUSolver(prob, restart=st.restart, precision=st.precision, preconditioner=st.preconditioner,
+                           poly_degree=st.poly_degree)
         if st.learned is not None:
             if st.preconditioner != "hybrid":
                 raise SimulationError("a learned smoother needs preconditioner='hybrid'")

@@ -303,7 +303,8 @@ class BasisSweep:
         if self.verbose:
             import sys
             print(f"sweep: GMRES f={freq:.6g} port {self.ports[port_index].port_number}: {info.iterations} iterations, "
-                  f"residual {info.residual:.2e}{' (plain fallback)' if info.reserved else ''}", file=sys.stderr,
+                  f"residual {info.residual:.2e}{' (plain smoother fallback)' if info.reserved & 1 else ''}"
+                  f"{' (polynomial dropped)' if info.reserved & 2 else ''}", file=sys.stderr,
                   flush=True)
         return x, info
 

@@ -116,6 +116,19 @@ int emg_solver_set_weights(emg_solver *s, const double *d_we, const double *d_wp
  * gradient correction is set and d_t is not NULL, d_t = diag(G^T K G)^{-1}
  * G^T r (np2), with the tables of the last emg_solver_set_matrix.           */
 int emg_solver_smoother_parts(emg_solver *s, const double *d_r, double *d_u, double *d_t, void *stream);
+/* Polynomial acceleration of the smoother (no counterpart in the reference,
+ * whose direct solvers need none): with degree d > 1 the preconditioner
+ * becomes M p(K M), 1 - z p(z) = prod_i (1 - z c_i), for the d complex128
+ * coefficients c_i (h_coef: 2 d doubles, host, copied; applied in order,
+ * normally the reciprocals of Leja-ordered harmonic Ritz values of K M).
+ * d = 1 switches it off.  Each preconditioner application then costs d - 1
+ * extra SpMVs and smoother applications.                                    */
+int emg_solver_set_poly(emg_solver *s, int degree, const double *h_coef);
+/* d (<= 16) steps of Arnoldi on K M (complex128, masked K, the smoother of the
+ * last emg_solver_set_matrix) from d_v0 (n complex128); h_out (host) receives
+ * the (d + 1) x d Hessenberg matrix column-major as complex128.  Uses the
+ * solver's Krylov storage: not during a solve.  Synchronises `stream`.       */
+int emg_solver_arnoldi_probe(emg_solver *s, int d, const double *d_v0, double *h_out, void *stream);
 /* d_x: initial guess in, solution out (n complex128). Synchronises `stream`. */
 int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, double tol, int maxiter, int max_refine,
                      emg_solve_info *info, void *stream);

@@ -55,3 +55,26 @@ def test_hybrid_cuts_iterations_same_solution(cuda, precision):
         print(pc, "iterations", out[pc][0], flush=True)
     assert out["hybrid"][0] * 3 < out["jacobi"][0]
     assert np.abs(out["hybrid"][1] - out["jacobi"][1]).max() < 1e-6
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_polynomial_cuts_steps_same_solution(cuda, precision):
+    """Degree-6 GMRES polynomial around the hybrid smoother: several times fewer GMRES steps
+    (each applying K and the smoother 6 times), the same FP64-certified S-parameters."""
+    print(f"polynomial vs plain hybrid precision={precision}", flush=True)
+    mesh, phys = _posts()
+    phys.set_frequency([3e9])
+    out = {}
+    for deg in (1, 6):
+        phys.solveroutine.precision = precision
+        phys.solveroutine.sweep = "plain"
+        phys.solveroutine.tol = 1e-9
+        phys.solveroutine.poly_degree = deg
+        data = phys.frequency_domain()
+        recs = data.item(0).solves
+        for r in recs:
+            assert r.converged and r.residual <= 1e-9, (deg, r)
+        out[deg] = (sum(r.iterations for r in recs), data.S_array()[0])
+        print("degree", deg, "GMRES steps", out[deg][0], flush=True)
+    assert out[6][0] * 2 < out[1][0]
+    assert np.abs(out[6][1] - out[1][1]).max() < 1e-6
