@@ -1204,12 +1204,11 @@ extern "C" int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, d
             const double rs = 1.0 / (rel * bn);
             k_c128_to_c64_scaled<<<1024, RB, 0, st>>>(n, s->r, rs, r32);
             EMG_HIP(hipMemsetAsync(d32, 0, n * 8, st));
-            static double floor_tol = -1.0;
-            if (floor_tol < 0) {
-                const char *e = getenv("EMG_INNER_FLOOR");
-                floor_tol = e ? atof(e) : 1e-7;
-            }
-            const double inner_tol = std::max(0.5 * tol / rel, floor_tol);
+            // A cycle stops once complex64 GMRES has reduced its residual 100-fold (or
+            // reached what tol needs): each FP64 update then starts a short cycle
+            // on a small Krylov basis (posts box, 201 points, polynomial degree 6:
+            // sweep 24.5 s with a 1e-7 floor, 15.9 s with 1e-2).
+            const double inner_tol = std::max(0.5 * tol / rel, 1e-2);
             int it;
             double est;
             rc = gmres_core<float>(s, s->vals32, s->inv32, r32, d32, inner_tol,

@@ -42,7 +42,7 @@ class GPUSolveSettings:
     def __init__(self, precision: int = 32, tol: float = 1e-8, restart: int = 150, maxiter: int = 60000,
                  max_refine: int = 0, warm_start: bool = True, store_fields: bool = True, sweep: str = "basis",
                  max_basis: int = 128, preconditioner: str = "hybrid", basis_tol: float = 1e-1,
-                 verbose: bool = False, learned=None, poly_degree: int = 1):
+                 verbose: bool = False, learned=None, poly_degree: int = 6):
         if sweep not in ("basis", "plain"):
             raise ValueError(f"sweep must be 'basis' or 'plain', not {sweep!r}")
         self.sweep = sweep

@@ -45,6 +45,7 @@ def test_hybrid_cuts_iterations_same_solution(cuda, precision):
     for pc in ("jacobi", "hybrid"):
         phys.solveroutine.precision = precision
         phys.solveroutine.preconditioner = pc
+        phys.solveroutine.poly_degree = 1
         phys.solveroutine.sweep = "plain"
         phys.solveroutine.tol = 1e-9
         data = phys.frequency_domain()
