@@ -176,6 +176,7 @@ def bench_sweep(args, dev):
             "max_fp64_residual": float(max(r.residual for r in recs)),
             "all_converged": bool(all(r.converged and r.residual <= args.sweep_tol for r in recs)),
             "basis_size": int(data.info.get("basis_size", 0)),
+            "poly_degree": args.poly_degree,
             "learned_check": data.info.get("learned"),
             "training": {"seconds": t_train, "frequencies_hz": train_f, "steps": args.learned_steps,
                          "loss_first": hist[0], "loss_last": hist[-1]},

@@ -282,24 +282,33 @@ class BasisSweep:
         if y0 is not None and self.m > 0:
             x0 = self.torch.empty(p.N, dtype=self.torch.complex128, device=p.dev)
             self._comb(self.V, self.m, y0, x0)
-        x, info = self.solver.solve(b, x0=x0, tol=self.tol * self.basis_tol, maxiter=self.maxiter,
-                                    max_refine=self.max_refine)
         if self.solver.learned is not None and self.learned_info is None:
-            # first GMRES of the sweep with learned weights: solve again with the plain smoother and
-            # keep the network only if it needed fewer iterations (the plain smoother is the fallback)
+            # first GMRES of the sweep with a learned smoother: solve with the plain smoother, then
+            # with the learned weights capped at the plain solve's iteration count, and keep the
+            # network only if it converged in fewer (the plain smoother is the fallback)
+            learned = self.solver.learned
+            self.solver.learned = None
             self.solver.set_weights(None, None)
             self.solver.set_matrix(p.K)
             x2, info2 = self.solver.solve(b, x0=x0, tol=self.tol * self.basis_tol, maxiter=self.maxiter,
                                           max_refine=self.max_refine)
-            use = bool(info.converged) and info.iterations < info2.iterations
+            self.solver.learned = learned
+            self.solver.set_matrix(p.K, freq)
+            cap = max(1, int(info2.iterations) - 1)
+            x, info = self.solver.solve(b, x0=x0, tol=self.tol * self.basis_tol, maxiter=cap,
+                                        max_refine=self.max_refine)
+            use = bool(info.converged) and not (info.reserved & 1) and info.iterations < info2.iterations
             self.learned_info = {"learned_iterations": int(info.iterations), "plain_iterations": int(info2.iterations),
                                  "used": use}
-            if use:
-                self.solver.set_matrix(p.K, freq)
-            else:
+            if not use:
                 self.solver.learned = None
+                self.solver.set_weights(None, None)
+                self.solver.set_matrix(p.K)
                 x, info2.iterations = x2, info2.iterations + info.iterations
                 info = info2
+        else:
+            x, info = self.solver.solve(b, x0=x0, tol=self.tol * self.basis_tol, maxiter=self.maxiter,
+                                        max_refine=self.max_refine)
         if self.verbose:
             import sys
             print(f"sweep: GMRES f={freq:.6g} port {self.ports[port_index].port_number}: {info.iterations} iterations, "
