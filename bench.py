@@ -289,6 +289,7 @@ def bench_eigen(args):
             "freqs_hz": [float(ds.freq) for ds in data.datasets],
             "relative_residuals": [float(ds.residual) for ds in data.datasets],
             "inner_solves": len(its), "inner_iterations_mean": float(np.mean(its)) if its else 0.0,
+            "inner_poly_degree": phys.solveroutine.poly_degree,
             "parity_vs_oracle": parity}
 
 

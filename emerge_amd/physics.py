@@ -239,7 +239,8 @@ class Electrodynamics3D:
         sigma = (2 * np.pi * f_t / C0) ** 2                                      # emfreq3d.py:838-839
         prob = self.problem()
         arn = ShiftInvertArnoldi(prob, sigma, ncv=ncv or max(4 * num_sols + 20, 24), inner_tol=inner_tol,
-                                 preconditioner=self.solveroutine.preconditioner, verbose=self.solveroutine.verbose)
+                                 preconditioner=self.solveroutine.preconditioner, verbose=self.solveroutine.verbose,
+                                 poly_degree=self.solveroutine.poly_degree)
         lam, X, resid = arn.run(num_sols, spurious_rel=spurious_rel)
         data = EMSimData()
         for i in range(lam.shape[0]):
