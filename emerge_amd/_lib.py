@@ -35,6 +35,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_csr_find": (ctypes.c_int, [I64, P, P, P, P, P, P]),
     "emg_scatter_add": (ctypes.c_int, [I64, P, P, F64, F64, P, P]),
     "emg_spmv": (ctypes.c_int, [I64, P, P, P, P, P, P, P]),
+    "emg_spmv_blocked": (ctypes.c_int, [I64, I64, P, P, P, P, P, P, P]),
     "emg_mdotc": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P]),
     "emg_maxpy": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P, P]),
     "emg_mcomb": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P]),

@@ -150,6 +150,84 @@ static void launch_spmv(hipStream_t st, int64_t n, const int32_t *rowptr, const 
         launch_spmv_l<R, false>(st, n, rowptr, cols, vals, mask, x, b, y);
 }
 
+// SpMV of a matrix on an entity pattern (emg_pattern_csr): DOF = mode * ne +
+// entity, and the rows (0, e) and (1, e) hold the same entities e'_j, row
+// (m, e) storing its (0, e'_j) entries at r_m + j and its (1, e'_j) entries at
+// r_m + deg + j, r_0 = rowptr[e], r_1 = r_0 + half (half = nnz / 2).  One
+// group of L lanes computes both rows of entity e: each column index and each
+// pair of x gathers serves four matrix entries.  Same masking as k_spmv.
+template <typename R, int L, bool MC>
+__global__ void __launch_bounds__(256) k_spmv_blk(int64_t ne, int64_t half, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ cols, const R *__restrict__ vals,
+                                                  const uint8_t *__restrict__ mask, const R *__restrict__ x,
+                                                  const R *__restrict__ b, R *__restrict__ y) {
+    const int64_t e = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / L;
+    const int lane = threadIdx.x % L;
+    if (e >= ne) return;
+    const bool k0 = mask == nullptr || mask[e], k1 = mask == nullptr || mask[ne + e];
+    cx<R> a0, a1;
+    if (k0 || k1) {
+        const int32_t r0 = rowptr[e];
+        const int32_t deg = (rowptr[e + 1] - r0) >> 1;
+        for (int32_t j = lane; j < deg; j += L) {
+            const int32_t c = cols[r0 + j];
+            if (c < 0 || c >= ne) continue;   // not an entity pattern: no out-of-range reads
+            cx<R> x0 = ld(x, c), x1 = ld(x, ne + c);
+            if (MC && mask != nullptr) {
+                if (!mask[c]) x0 = cx<R>();
+                if (!mask[ne + c]) x1 = cx<R>();
+            }
+            const int64_t p = r0 + j;
+            a0 = cadd(a0, cadd(cmul(ld(vals, p), x0), cmul(ld(vals, p + deg), x1)));
+            a1 = cadd(a1, cadd(cmul(ld(vals, half + p), x0), cmul(ld(vals, half + p + deg), x1)));
+        }
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+        a0.re += __shfl_xor(a0.re, off, L);
+        a0.im += __shfl_xor(a0.im, off, L);
+        a1.re += __shfl_xor(a1.re, off, L);
+        a1.im += __shfl_xor(a1.im, off, L);
+    }
+    if (lane == 0) {
+        cx<R> v0 = k0 ? a0 : ld(x, e), v1 = k1 ? a1 : ld(x, ne + e);
+        if (b != nullptr) {
+            const cx<R> b0 = ld(b, e), b1 = ld(b, ne + e);
+            v0 = cx<R>(b0.re - v0.re, b0.im - v0.im);
+            v1 = cx<R>(b1.re - v1.re, b1.im - v1.im);
+        }
+        st(y, e, v0);
+        st(y, ne + e, v1);
+    }
+}
+
+// Lanes per entity row of the blocked SpMV: EMG_SPMV_BLK_LANES (8, 16 or 32) if
+// set, else 16 (posts-box sweep: 4 lanes 22.1 s, 8 lanes 14.0 s, 16 lanes 12.3 s).
+static int spmv_blk_lanes() {
+    static int lanes = 0;
+    if (lanes == 0) {
+        const char *e = getenv("EMG_SPMV_BLK_LANES");
+        const int v = e ? atoi(e) : 16;
+        lanes = (v == 8 || v == 16 || v == 32) ? v : 16;
+    }
+    return lanes;
+}
+
+template <typename R>
+static void launch_spmv_blk(hipStream_t st, int64_t n, int64_t nnz, const int32_t *rowptr, const int32_t *cols,
+                            const R *vals, const uint8_t *mask, const R *x, const R *b, R *y, bool mask_cols = true) {
+    const int64_t ne = n / 2, half = nnz / 2;
+    const int L = spmv_blk_lanes();
+    const unsigned grid = (unsigned)((ne * L + 255) / 256);
+#define EMG_BLK(LL, MCV) k_spmv_blk<R, LL, MCV><<<grid, 256, 0, st>>>(ne, half, rowptr, cols, vals, mask, x, b, y)
+    if (mask_cols) {
+        if (L == 8) EMG_BLK(8, true); else if (L == 32) EMG_BLK(32, true); else EMG_BLK(16, true);
+    } else {
+        if (L == 8) EMG_BLK(8, false); else if (L == 32) EMG_BLK(32, false); else EMG_BLK(16, false);
+    }
+#undef EMG_BLK
+}
+
 // ---------------------------------------------------------------- BLAS-1
 static constexpr int RB = 256;   // reduction block
 static constexpr int DT = 8;     // dot products per tile
@@ -658,7 +736,7 @@ static void apply_prec_poly(emg_solver *s, const R *vals, const R *inv, const R 
     const R *q = x;
     for (int i = 0; i < s->poly_deg - 1; i++) {
         apply_prec<R>(s, inv, q, pu, st);
-        launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, pu, nullptr, pt, sizeof(R) == 8);
+        launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, pu, nullptr, pt, sizeof(R) == 8);
         k_poly_step<R><<<g, RB, 0, st>>>(n, s->poly_c, i, q, pt, pp, py);
         q = pp;
     }
@@ -846,7 +924,7 @@ extern "C" int emg_solver_arnoldi_probe(emg_solver *s, int d, const double *d_v0
         const int nv = j + 1, nbm = mdot_blocks(n, nv);
         const dim3 grid(nbm, (nv + DT - 1) / DT);
         apply_prec<double>(s, s->inv64, vj, z, st);
-        launch_spmv<double>(st, n, s->rowptr, s->cols, s->vals, s->mask, z, nullptr, w);
+        launch_spmv_blk<double>(st, n, s->nnz, s->rowptr, s->cols, s->vals, s->mask, z, nullptr, w);
         double wn = 0;
         for (int pass = 0; pass < 2; pass++) {
             k_mdot<double><<<grid, RB, 0, st>>>(n, nv, V, n, w, s->partial, nullptr);
@@ -1027,7 +1105,7 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
     const dim3 grid(nbm, (nv + DT - 1) / DT);
     apply_prec_poly<R>(s, vals, inv, vj, z, st);
     // the complex64 copy has its masked columns zeroed (k_to_c64_masked)
-    launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn, sizeof(R) == 8);
+    launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn, sizeof(R) == 8);
     k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, nullptr);
     k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, h1, 0, nullptr);
     k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, h1, vn, s->pn, nullptr);
@@ -1103,7 +1181,7 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     double resid = 1.0;
     while (true) {
         // r = rhs - K u  ->  v0 = r / |r|
-        launch_spmv<R>(st, n, s->rowptr, s->cols, vals, s->mask, u, rhs, w, sizeof(R) == 8);
+        launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, u, rhs, w, sizeof(R) == 8);
         k_norm2<R><<<nb, RB, 0, st>>>(n, w, s->partial);
         k_scale_by_norm<R><<<nb, RB, 0, st>>>(n, s->partial, nb, w, V, s->nrm);
         double beta;
@@ -1165,7 +1243,7 @@ extern "C" int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, d
     memset(info, 0, sizeof(*info));
     auto true_resid = [&](double *out_rel, double *bn) -> int {
         // r = b - K x in complex128 (masked)
-        launch_spmv<double>(st, n, s->rowptr, s->cols, s->vals, s->mask, d_x, d_b, s->r);
+        launch_spmv_blk<double>(st, n, s->nnz, s->rowptr, s->cols, s->vals, s->mask, d_x, d_b, s->r);
         k_norm2<double><<<nb, RB, 0, st>>>(n, s->r, s->partial);
         k_norm2<double><<<nb, RB, 0, st>>>(n, d_b, s->partial + 1024);
         std::vector<double> p(2048);
@@ -1264,6 +1342,16 @@ extern "C" int emg_spmv(int64_t n, const int32_t *d_rowptr, const int32_t *d_col
                         const uint8_t *d_mask, const double *d_x, double *d_y, void *stream) {
     EMG_CHECK(d_rowptr && d_cols && d_vals && d_x && d_y, "emg_spmv: null argument");
     launch_spmv<double>((hipStream_t)stream, n, d_rowptr, d_cols, d_vals, d_mask, d_x, nullptr, d_y);
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int emg_spmv_blocked(int64_t n, int64_t nnz, const int32_t *d_rowptr, const int32_t *d_cols,
+                                const double *d_vals, const uint8_t *d_mask, const double *d_x, double *d_y,
+                                void *stream) {
+    EMG_CHECK(d_rowptr && d_cols && d_vals && d_x && d_y, "emg_spmv_blocked: null argument");
+    EMG_CHECK(n > 0 && n % 2 == 0 && nnz % 2 == 0, "emg_spmv_blocked: n and nnz must be even");
+    launch_spmv_blk<double>((hipStream_t)stream, n, nnz, d_rowptr, d_cols, d_vals, d_mask, d_x, nullptr, d_y);
     EMG_LAUNCH_CHECK();
     return 0;
 }

@@ -148,6 +148,13 @@ int emg_scatter_add(int64_t m, const int32_t *d_pos, const double *d_v, double g
  * back to back; d_work is scratch of ceil(nv/8)*1024*16 + 1024 doubles.    */
 int emg_spmv(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_vals, const uint8_t *d_mask,
              const double *d_x, double *d_y, void *stream);
+/* The same product for a matrix on an entity pattern (the CSR of
+ * emg_pattern_csr: rows (0, e) and (1, e) share their column entities, the
+ * mode-1 columns of a row follow its mode-0 columns): both rows of an entity in
+ * one pass, each column index read once per four entries.  nnz = rowptr[n].
+ * No counterpart in the reference (scipy's CSR product).                     */
+int emg_spmv_blocked(int64_t n, int64_t nnz, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_vals,
+                     const uint8_t *d_mask, const double *d_x, double *d_y, void *stream);
 int emg_mdotc(int64_t n, int nv, const double *d_V, const double *d_w, double *d_h, double *d_work, void *stream);
 int emg_maxpy(int64_t n, int nv, const double *d_V, const double *d_h, double *d_w, double *d_norm, double *d_work,
               void *stream);
