@@ -1217,8 +1217,9 @@ extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, c
             EMG_HIP(hipGetDevice(&dev));
             EMG_HIP(hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev));
         }
-        // 4 resident workgroups per CU (register bound: 4 waves per SIMD), two rounds of them
-        const int64_t grid = std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * 8);
+        // 4 resident workgroups per CU (register bound: 4 waves per SIMD); 8 rounds of them balance
+        // the chunks' uneven costs better than 2 (C2: 3.68 -> 3.55 ms; 1, 3, 12 and 16 rounds measured too)
+        const int64_t grid = std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * 32);
         const int efm = (flags & EMG_EXACT_FACE_MASS) ? 1 : 0;
         const cplx *ce = reinterpret_cast<const cplx *>(d_eps), *cn = reinterpret_cast<const cplx *>(d_nu);
         cplx *oE = reinterpret_cast<cplx *>(d_E), *oB = reinterpret_cast<cplx *>(d_B);
