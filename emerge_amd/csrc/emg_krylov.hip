@@ -509,13 +509,25 @@ __global__ void __launch_bounds__(RB) k_bj_grad_apply(int64_t n_ent, const R *__
     const cx<R> r0 = ld(r, e), r1 = ld(r, e + n_ent);
     cx<R> z0 = cadd(cmul(a, r0), cmul(b, r1));
     cx<R> z1 = cadd(cmul(c, r0), cmul(d, r1));
-    for (int32_t q = grp[e]; q < grp[e + 1]; q++) {
-        const cx<R> tv = ld(t, gcol[q]);
-        z0 = cx<R>(z0.re + gv[q] * tv.re, z0.im + gv[q] * tv.im);
-    }
-    for (int32_t q = grp[e + n_ent]; q < grp[e + n_ent + 1]; q++) {
-        const cx<R> tv = ld(t, gcol[q]);
-        z1 = cx<R>(z1.re + gv[q] * tv.re, z1.im + gv[q] * tv.im);
+    const int32_t q0 = grp[e], q1 = grp[e + 1], q2 = grp[e + n_ent], q3 = grp[e + n_ent + 1];
+    if (q1 - q0 == q3 - q2) {
+        // both modes' rows list the same P2 nodes in the same order (the entity's nodes less the
+        // dropped ones): one column index and one gather of t serve both rows
+        for (int32_t k = 0; k < q1 - q0; k++) {
+            const cx<R> tv = ld(t, gcol[q0 + k]);
+            const R g0 = gv[q0 + k], g1 = gv[q2 + k];
+            z0 = cx<R>(z0.re + g0 * tv.re, z0.im + g0 * tv.im);
+            z1 = cx<R>(z1.re + g1 * tv.re, z1.im + g1 * tv.im);
+        }
+    } else {
+        for (int32_t q = q0; q < q1; q++) {
+            const cx<R> tv = ld(t, gcol[q]);
+            z0 = cx<R>(z0.re + gv[q] * tv.re, z0.im + gv[q] * tv.im);
+        }
+        for (int32_t q = q2; q < q3; q++) {
+            const cx<R> tv = ld(t, gcol[q]);
+            z1 = cx<R>(z1.re + gv[q] * tv.re, z1.im + gv[q] * tv.im);
+        }
     }
     st(z, e, z0);
     st(z, e + n_ent, z1);
