@@ -253,6 +253,7 @@ class GPUSolver:
                    "emg_solver_create")
         self.h = h
         self.precision = precision
+        self.restart = restart
         self.preconditioner = preconditioner
         if preconditioner == "hybrid":
             G = problem.gradient()
@@ -302,6 +303,9 @@ class GPUSolver:
             v = torch.randn((self.p.N, 2), generator=g, dtype=torch.float64, device=self.p.dev)
             self._v0 = torch.view_as_complex(v).contiguous() * self.p.mask.to(torch.float64)
         _lib.check(self.L.emg_solver_set_poly(self.h, 1, None), "emg_solver_set_poly")
+        self.poly_active = 1
+        if (d + 1) * 16 > (self.restart + 1) * (16 if self.precision == 64 else 8):
+            return                                          # the probe needs d + 1 complex128 basis slots
         H = np.zeros((d, d + 1), dtype=np.complex128)     # column-major (d+1) x d
         _lib.check(self.L.emg_solver_arnoldi_probe(self.h, d, _lib.ptr(self._v0), H.ctypes.data, _lib.stream_ptr()),
                    "emg_solver_arnoldi_probe")
