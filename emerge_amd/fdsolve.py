@@ -307,8 +307,12 @@ class GPUSolver:
         if (d + 1) * 16 > (self.restart + 1) * (16 if self.precision == 64 else 8):
             return                                          # the probe needs d + 1 complex128 basis slots
         H = np.zeros((d, d + 1), dtype=np.complex128)     # column-major (d+1) x d
-        _lib.check(self.L.emg_solver_arnoldi_probe(self.h, d, _lib.ptr(self._v0), H.ctypes.data, _lib.stream_ptr()),
-                   "emg_solver_arnoldi_probe")
+        rc = self.L.emg_solver_arnoldi_probe(self.h, d, _lib.ptr(self._v0), H.ctypes.data, _lib.stream_ptr())
+        if rc != 0:
+            msg = self.L.emg_last_error().decode(errors="replace")
+            if "breakdown" not in msg:
+                _lib.check(rc, "emg_solver_arnoldi_probe")
+            return                                          # Krylov space exhausted: the smoother alone
         c = poly_coefficients(H.T)
         if c is None:
             self.poly_active = 1
