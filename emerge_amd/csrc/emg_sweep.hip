@@ -41,16 +41,20 @@ __device__ __forceinline__ void stc(double *p, int64_t i, c2 v) {
 }
 
 constexpr int RES_BLOCK = 256;
-constexpr int ROW_LANES = 64;        // one wavefront per CSR row
+constexpr int ROW_LANES = 64;        // one wavefront per entity (two CSR rows)
 constexpr int MAX_VEC = 16;
 constexpr int RES_GRID_MAX = 4096;   // persistent grid: partial sums per block
 
-// One wavefront per row; lane l handles vector j = l % NV over the row's
-// entries l / NV, l / NV + 64 / NV, ...  E, B and the column index of an entry
-// are loaded once per group of NV lanes and shared by the NV vectors.
+// One wavefront per entity e of the pattern (emg_pattern_csr: DOF = mode * ne +
+// entity; rows (0, e) and (1, e) list the same column entities e'_k, a row's
+// (1, e'_k) entries deg after its (0, e'_k) ones, row (1, e) half = rowptr[ne]
+// entries after row (0, e)).  Lane l handles vector j = l % NV over the
+// column entities l / NV, l / NV + 64 / NV, ...: the two NV-vector runs of x
+// gathered for e'_k serve four matrix entries, and E, B and the column index
+// of an entry are loaded once per group of NV lanes and shared by the NV vectors.
 template <int NV>
 __global__ void __launch_bounds__(RES_BLOCK)
-    k_sweep_residual(int64_t n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
+    k_sweep_residual(int64_t ne, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ cols,
                      const double *__restrict__ E, const double *__restrict__ B, const uint8_t *__restrict__ mask,
                      const double *__restrict__ s, const double *__restrict__ X, const double *__restrict__ Q,
                      const int32_t *__restrict__ qidx, const double *__restrict__ c,
@@ -63,43 +67,54 @@ __global__ void __launch_bounds__(RES_BLOCK)
     const double sj = s[j];
     const int qj = qidx[j];
     const c2 cj = ldc(c, j);
+    const int64_t n = 2 * ne;
+    const int64_t half = rowptr[ne];
     double rr = 0.0, bb = 0.0;
     const int64_t groups = (int64_t)gridDim.x * (RES_BLOCK / ROW_LANES);
-    for (int64_t row = blockIdx.x * (int64_t)(RES_BLOCK / ROW_LANES) + threadIdx.x / ROW_LANES; row < n;
-         row += groups) {
-        double are = 0.0, aim = 0.0;
-        const int32_t p0 = rowptr[row], p1 = rowptr[row + 1];
-        for (int32_t p = p0 + part; p < p1; p += SPLIT) {
-            const int32_t col = cols[p];
-            const c2 e = ldc(E, p);
-            const c2 b = ldc(B, p);
-            const c2 x = ldc(X, (int64_t)col * NV + j);
-            const double kre = e.re - sj * b.re, kim = e.im - sj * b.im;
-            are += kre * x.re - kim * x.im;
-            aim += kre * x.im + kim * x.re;
+    for (int64_t e = blockIdx.x * (int64_t)(RES_BLOCK / ROW_LANES) + threadIdx.x / ROW_LANES; e < ne; e += groups) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};   // row (0, e) re, im; row (1, e) re, im
+        const int32_t r0 = rowptr[e];
+        const int32_t deg = (rowptr[e + 1] - r0) >> 1;
+        for (int32_t k = part; k < deg; k += SPLIT) {
+            const int32_t col = cols[r0 + k];
+            if (col < 0 || col >= ne) continue;   // not an entity pattern: no out-of-range reads
+            const c2 x0 = ldc(X, (int64_t)col * NV + j), x1 = ldc(X, (int64_t)(ne + col) * NV + j);
+            const int64_t pos[4] = {r0 + k, r0 + k + deg, half + r0 + k, half + r0 + k + deg};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const c2 ev = ldc(E, pos[q]), bv = ldc(B, pos[q]);
+                const c2 x = (q & 1) ? x1 : x0;
+                const double kre = ev.re - sj * bv.re, kim = ev.im - sj * bv.im;
+                acc[(q >> 1) * 2] += kre * x.re - kim * x.im;
+                acc[(q >> 1) * 2 + 1] += kre * x.im + kim * x.re;
+            }
         }
 #pragma unroll
-        for (int off = ROW_LANES / 2; off >= NV; off >>= 1) {
-            are += __shfl_xor(are, off, ROW_LANES);
-            aim += __shfl_xor(aim, off, ROW_LANES);
-        }
+        for (int off = ROW_LANES / 2; off >= NV; off >>= 1)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] += __shfl_xor(acc[q], off, ROW_LANES);
         if (part == 0) {
-            const int32_t pr = prow_of_row ? prow_of_row[row] : -1;
-            if (pr >= 0) {
-                const c2 z = ldc(Zp, (int64_t)pr * NV + j);
-                are += z.re;
-                aim += z.im;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int64_t row = h * ne + e;
+                double are = acc[2 * h], aim = acc[2 * h + 1];
+                const int32_t pr = prow_of_row ? prow_of_row[row] : -1;
+                if (pr >= 0) {
+                    const c2 z = ldc(Zp, (int64_t)pr * NV + j);
+                    are += z.re;
+                    aim += z.im;
+                }
+                double rre = 0.0, rim = 0.0;
+                if (mask == nullptr || mask[row]) {
+                    const c2 q = ldc(Q, (int64_t)qj * n + row);
+                    const double bre = cj.re * q.re - cj.im * q.im, bim = cj.re * q.im + cj.im * q.re;
+                    rre = bre - are;
+                    rim = bim - aim;
+                    rr += rre * rre + rim * rim;
+                    bb += bre * bre + bim * bim;
+                }
+                if (R != nullptr) stc(R, row * NV + j, c2{rre, rim});
             }
-            double rre = 0.0, rim = 0.0;
-            if (mask == nullptr || mask[row]) {
-                const c2 q = ldc(Q, (int64_t)qj * n + row);
-                const double bre = cj.re * q.re - cj.im * q.im, bim = cj.re * q.im + cj.im * q.re;
-                rre = bre - are;
-                rim = bim - aim;
-                rr += rre * rre + rim * rim;
-                bb += bre * bre + bim * bim;
-            }
-            if (R != nullptr) stc(R, row * NV + j, c2{rre, rim});
         }
     }
     // block reduction of (rr, bb) per vector
@@ -186,7 +201,7 @@ void launch_residual(hipStream_t st, int grid, int64_t n, const int32_t *rowptr,
                      const double *B, const uint8_t *mask, const double *s, const double *X, const double *Q,
                      const int32_t *qidx, const double *c, const int32_t *prow, const double *Zp, double *R,
                      double *partial) {
-    k_sweep_residual<NV><<<grid, RES_BLOCK, 0, st>>>(n, rowptr, cols, E, B, mask, s, X, Q, qidx, c, prow, Zp, R,
+    k_sweep_residual<NV><<<grid, RES_BLOCK, 0, st>>>(n / 2, rowptr, cols, E, B, mask, s, X, Q, qidx, c, prow, Zp, R,
                                                      partial);
 }
 
@@ -209,8 +224,9 @@ extern "C" int emg_sweep_residual(int64_t n, const int32_t *d_rowptr, const int3
     EMG_CHECK(valid_nvec(nvec), "emg_sweep_residual: nvec must be 1, 2, 4, 8 or 16");
     EMG_CHECK((d_prow_of_row == nullptr) == (d_Zp == nullptr), "emg_sweep_residual: port rows need Zp");
     hipStream_t st = (hipStream_t)stream;
-    constexpr int rows_per_block = RES_BLOCK / ROW_LANES;
-    const int grid = (int)std::min<int64_t>(RES_GRID_MAX, (n + rows_per_block - 1) / rows_per_block);
+    EMG_CHECK(n % 2 == 0, "emg_sweep_residual: n must be 2 * entities (an entity pattern)");
+    constexpr int ents_per_block = RES_BLOCK / ROW_LANES;
+    const int grid = (int)std::min<int64_t>(RES_GRID_MAX, (n / 2 + ents_per_block - 1) / ents_per_block);
     switch (nvec) {
         case 1: launch_residual<1>(st, grid, n, d_rowptr, d_cols, d_E, d_B, d_mask, d_s, d_X, d_Q, d_qidx, d_c, d_prow_of_row, d_Zp, d_R, d_work); break;
         case 2: launch_residual<2>(st, grid, n, d_rowptr, d_cols, d_E, d_B, d_mask, d_s, d_X, d_Q, d_qidx, d_c, d_prow_of_row, d_Zp, d_R, d_work); break;

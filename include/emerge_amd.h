@@ -200,7 +200,9 @@ int emg_interp_points(int64_t npts, const int32_t *d_ptet, const double *d_bary,
  *   d_s (nvec) f64 = k0^2; d_Q (nq, n) c128 port sources for force scale 1;
  *   d_c (nvec) c128 force scales; d_prow_of_row (n) int32 index into Zp or -1
  *   (NULL together with d_Zp when there are no Robin rows); d_R (n, nvec) or
- *   NULL; d_work >= 4096 * nvec * 2 doubles.                                    */
+ *   NULL; d_work >= 4096 * nvec * 2 doubles.  E and B must be on an entity
+ *   pattern (emg_pattern_csr), n = 2 * entities: both rows of an entity are
+ *   formed together.                                                           */
 int emg_sweep_residual(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_E,
                        const double *d_B, const uint8_t *d_mask, int nvec, const double *d_s, const double *d_X,
                        const double *d_Q, const int32_t *d_qidx, const double *d_c, const int32_t *d_prow_of_row,
