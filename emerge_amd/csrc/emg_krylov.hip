@@ -168,7 +168,8 @@ __global__ void __launch_bounds__(256) k_spmv_blk(int64_t ne, int64_t half, cons
     cx<R> a0, a1;
     if (k0 || k1) {
         const int32_t r0 = rowptr[e];
-        const int32_t deg = (rowptr[e + 1] - r0) >> 1;
+        // a row beyond the first half is not an entity pattern: read nothing past nnz
+        const int32_t deg = (rowptr[e + 1] <= half && rowptr[e + 1] >= r0) ? (rowptr[e + 1] - r0) >> 1 : 0;
         for (int32_t j = lane; j < deg; j += L) {
             const int32_t c = cols[r0 + j];
             if (c < 0 || c >= ne) continue;   // not an entity pattern: no out-of-range reads
@@ -778,6 +779,36 @@ struct arn_off {
 };
 static constexpr int CHUNK_IT = 10;   // Arnoldi steps per captured graph
 
+// Sets *bad when the CSR is not on an entity pattern (the layout k_spmv_blk and
+// the smoother rely on): rows (0, e) and (1, e) list the same column entities
+// c_j < ne, the (1, c_j) entries of a row follow its (0, c_j) entries, and row
+// (1, e) starts half = nnz / 2 entries after row (0, e).
+__global__ void __launch_bounds__(256) k_check_entity_pattern(int64_t ne, int64_t nnz,
+                                                              const int32_t *__restrict__ rowptr,
+                                                              const int32_t *__restrict__ cols, int *bad) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e > ne) return;
+    const int64_t half = nnz / 2;
+    if (e == ne) {
+        if (rowptr[0] != 0 || rowptr[ne] != half || rowptr[2 * ne] != nnz || (nnz & 1)) atomicOr(bad, 1);
+        return;
+    }
+    const int64_t r0 = rowptr[e], r1 = rowptr[e + 1];
+    if (r1 < r0 || r1 > half || ((r1 - r0) & 1) || rowptr[ne + e] != r0 + half || rowptr[ne + e + 1] != r1 + half) {
+        atomicOr(bad, 1);
+        return;
+    }
+    const int64_t deg = (r1 - r0) / 2;
+    for (int64_t j = 0; j < deg; j++) {
+        const int32_t c = cols[r0 + j];
+        if (c < 0 || c >= ne || cols[r0 + deg + j] != c + ne || cols[half + r0 + j] != c ||
+            cols[half + r0 + deg + j] != c + ne) {
+            atomicOr(bad, 1);
+            return;
+        }
+    }
+}
+
 extern "C" int emg_solver_create(int64_t n, int64_t nnz, const int32_t *d_rowptr, const int32_t *d_cols, int restart,
                                  int precision, emg_solver **out) {
     EMG_CHECK(out && d_rowptr && d_cols, "emg_solver_create: null argument");
@@ -810,6 +841,30 @@ extern "C" int emg_solver_create(int64_t n, int64_t nnz, const int32_t *d_rowptr
         emg::set_error(std::string("emg_solver_create: ") + hipGetErrorString(e));
         free_solver(s);
         return -2;
+    }
+    // the blocked SpMV and the 2x2 block smoother need an entity pattern: check it once
+    int bad = 0;
+    if (nnz <= 0 || nnz >= ((int64_t)1 << 31)) bad = 1;
+    if (!bad) {
+        int *d_bad = reinterpret_cast<int *>(s->flag);
+        e = hipMemset(d_bad, 0, sizeof(int));
+        if (e == hipSuccess) {
+            k_check_entity_pattern<<<emg::grid_for(s->n_ent + 1, 256), 256>>>(s->n_ent, nnz, d_rowptr, d_cols, d_bad);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            emg::set_error(std::string("emg_solver_create: pattern check: ") + hipGetErrorString(e));
+            free_solver(s);
+            return -2;
+        }
+    }
+    if (bad) {
+        emg::set_error("emg_solver_create: the CSR is not on an entity pattern (emg_pattern_csr layout: rows e and "
+                       "e + n/2 share their column entities, mode-1 columns after mode-0 ones, row e + n/2 starting "
+                       "nnz/2 after row e); pass the full unreduced matrix and the PEC mask");
+        free_solver(s);
+        return -1;
     }
     *out = s;
     return 0;
@@ -979,6 +1034,12 @@ extern "C" int emg_solver_set_gradient(emg_solver *s, int64_t np2, const int32_t
     EMG_HIP(hipStreamSynchronize(st));
     EMG_CHECK(nnz_g == nnz_gt && nnz_g > 0, "emg_solver_set_gradient: G and G^T entry counts differ");
     EMG_CHECK(nnz_g <= G_ROW_MAX * s->n, "emg_solver_set_gradient: more than 8 entries per row of G");
+    // captured step graphs hold the old gradient buffers: drop them before the buffers go
+    EMG_HIP(hipStreamSynchronize(st));
+    for (auto &g : s->gs) {
+        for (hipGraphExec_t e : g.exec) (void)hipGraphExecDestroy(e);
+        g.exec.clear();
+    }
     void *old[] = {s->g_vals32, s->gt_vals32, s->dS, s->dsinv64, s->tg, s->dsinv32};
     for (void *p : old) (void)hipFree(p);
     s->g_vals32 = s->gt_vals32 = nullptr;

@@ -40,6 +40,17 @@ from .emdata import SolveRecord
 from .fdsolve import tri_dofs
 
 MAX_BLOCK = 16
+MAX_BASIS = 128           # emg_basis_combine takes nvec * m <= 2048 with nvec = 16
+MAX_MULTI = 512           # vectors per emg_mdotc / emg_maxpy call
+
+
+class StoredSolution:
+    """A GMRES solution kept outside the basis (the basis was full): its vector
+    and the port-point fields sampled from it."""
+
+    def __init__(self, x, fields):
+        self.x = x
+        self.fields = fields
 
 
 def _c128_to_f64(a: np.ndarray) -> np.ndarray:
@@ -62,6 +73,8 @@ class BasisSweep:
         self.maxiter = maxiter
         self.max_refine = max_refine
         self.L = _lib.load()
+        if not 1 <= int(max_basis) <= MAX_BASIS:
+            raise ValueError(f"max_basis must be in [1, {MAX_BASIS}], not {max_basis!r}")
         self.max_basis = max_basis
         self.block = block
         p, dev, N = problem, problem.dev, problem.N
@@ -69,9 +82,9 @@ class BasisSweep:
         self.V = torch.zeros((max_basis, N), dtype=torch.complex128, device=dev)
         self.m = 0
         self.Fb: list[np.ndarray] = []                     # port-point fields of each basis vector
-        self.work = torch.empty(max(((max_basis + 8) // 8) * 1024 * 16 + 1024, 4096 * MAX_BLOCK * 2),
+        self.work = torch.empty(max(((MAX_MULTI + 8) // 8) * 1024 * 16 + 1024, 4096 * MAX_BLOCK * 2),
                                 dtype=torch.float64, device=dev)
-        self.h = torch.empty(2 * max_basis * MAX_BLOCK, dtype=torch.float64, device=dev)
+        self.h = torch.empty(2 * max(max_basis * MAX_BLOCK, MAX_MULTI), dtype=torch.float64, device=dev)
         self.nrm = torch.empty(2 * MAX_BLOCK, dtype=torch.float64, device=dev)
         self.x = torch.empty(N, dtype=torch.complex128, device=dev)
         # port sources q_p (force scale 1), masked
@@ -143,17 +156,24 @@ class BasisSweep:
                                    _lib.ptr(x), _lib.ptr(y), _lib.stream_ptr()), "emg_spmv")
 
     def _mdotc(self, V, nv, w) -> np.ndarray:
-        """conj(V_i) . w for i < nv."""
-        _lib.check(self.L.emg_mdotc(self.p.N, nv, _lib.ptr(V), _lib.ptr(w), _lib.ptr(self.h), _lib.ptr(self.work),
-                                    _lib.stream_ptr()), "emg_mdotc")
-        hv = self.h[: 2 * nv].cpu().numpy()
-        return hv[0::2] + 1j * hv[1::2]
+        """conj(V_i) . w for i < nv (in calls of at most MAX_MULTI vectors)."""
+        out = np.zeros(nv, dtype=np.complex128)
+        for i0 in range(0, nv, MAX_MULTI):
+            k = min(MAX_MULTI, nv - i0)
+            _lib.check(self.L.emg_mdotc(self.p.N, k, _lib.ptr(V[i0:i0 + k]), _lib.ptr(w), _lib.ptr(self.h),
+                                        _lib.ptr(self.work), _lib.stream_ptr()), "emg_mdotc")
+            hv = self.h[: 2 * k].cpu().numpy()
+            out[i0:i0 + k] = hv[0::2] + 1j * hv[1::2]
+        return out
 
     def _maxpy(self, V, nv, coef: np.ndarray, w) -> float:
-        """w -= sum_i coef_i V_i; returns ||w||."""
-        self.h[: 2 * nv].copy_(self.torch.from_numpy(_c128_to_f64(coef)))
-        _lib.check(self.L.emg_maxpy(self.p.N, nv, _lib.ptr(V), _lib.ptr(self.h), _lib.ptr(w), _lib.ptr(self.nrm),
-                                    _lib.ptr(self.work), _lib.stream_ptr()), "emg_maxpy")
+        """w -= sum_i coef_i V_i; returns ||w|| (in calls of at most MAX_MULTI vectors)."""
+        for i0 in range(0, max(nv, 1), MAX_MULTI):
+            k = min(MAX_MULTI, nv - i0)
+            if k > 0:
+                self.h[: 2 * k].copy_(self.torch.from_numpy(_c128_to_f64(coef[i0:i0 + k])))
+            _lib.check(self.L.emg_maxpy(self.p.N, k, _lib.ptr(V[i0:i0 + max(k, 1)]), _lib.ptr(self.h), _lib.ptr(w),
+                                        _lib.ptr(self.nrm), _lib.ptr(self.work), _lib.stream_ptr()), "emg_maxpy")
         return float(self.nrm[0].item())
 
     def _comb(self, V, nv, coef: np.ndarray, out):
@@ -162,8 +182,9 @@ class BasisSweep:
                    "emg_mcomb")
 
     # ------------------------------------------------------------ basis
-    def add(self, x) -> np.ndarray:
-        """Orthonormalise x into the basis; returns y with x = V[:m] y (m after the call)."""
+    def add(self, x):
+        """Orthonormalise x into the basis; returns y with x = V[:m] y (m after the call), or a
+        StoredSolution holding x itself when the basis is full and x has a component outside it."""
         m = self.m
         w = self.x
         w.copy_(x)
@@ -175,8 +196,11 @@ class BasisSweep:
                 h = self._mdotc(self.V, m, w)
                 r = self._maxpy(self.V, m, h, w)
                 y += h
-        if m >= self.max_basis or r <= 1e-13 * xn:
+        if r <= 1e-13 * xn:
             return y
+        if m >= self.max_basis:
+            xs = x.clone()
+            return StoredSolution(xs, self.sampler.fields(xs[None, :])[0])
         self._comb(w[None, :], 1, np.array([1.0 / r], dtype=np.complex128), self.V[m])
         v = self.V[m]
         # the columns of K(f) v for every frequency: E v, B v, P_t v (masked operator)
@@ -362,11 +386,15 @@ class BasisSweep:
                       f"basis {self.m}, GMRES solves {nfull}", file=sys.stderr, flush=True)
         return out
 
-    def fields_of(self, y: np.ndarray) -> np.ndarray:
+    def fields_of(self, y) -> np.ndarray:
+        if isinstance(y, StoredSolution):
+            return y.fields
         Fb = np.stack(self.Fb[: y.shape[0]])                   # (m, npts, 3)
         return np.tensordot(y, Fb, axes=(0, 0))
 
-    def vector_of(self, y: np.ndarray):
+    def vector_of(self, y):
+        if isinstance(y, StoredSolution):
+            return y.x
         x = self.torch.empty(self.p.N, dtype=self.torch.complex128, device=self.p.dev)
         self._comb(self.V, y.shape[0], y, x)
         return x

@@ -107,3 +107,20 @@ def test_blocked_spmv_matches_csr_spmv(cuda):
         err = float((y1 - y2).abs().max() / y1.abs().max())
         print("blocked vs csr", "masked" if mask is not None else "plain", err, flush=True)
         assert err < 1e-14
+
+
+def test_solver_rejects_csr_off_the_entity_pattern(cuda):
+    """A PEC-reduced or re-sorted CSR breaks the blocked SpMV's layout: emg_solver_create refuses it."""
+    import ctypes
+    import torch
+    import scipy.sparse as sp
+    from emerge_amd import _lib
+    A = sp.random(40, 40, density=0.2, random_state=1, format="csr") + sp.eye(40, format="csr")
+    A.sort_indices()
+    d = lambda a: torch.from_numpy(a.astype(np.int32)).to(cuda)
+    rp, ci = d(A.indptr), d(A.indices)
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    rc = L.emg_solver_create(40, A.nnz, _lib.ptr(rp), _lib.ptr(ci), 20, 64, ctypes.byref(h))
+    assert rc != 0
+    assert b"entity pattern" in L.emg_last_error()

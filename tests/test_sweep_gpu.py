@@ -154,3 +154,36 @@ def test_basis_sweep_lumped_abc_matches_direct(cuda):
     x = data.item(2)._fields[1]
     assert x.shape == (phys.problem().N,)
     assert np.all(x[~phys.problem().mask.cpu().numpy().astype(bool)] == 0)
+
+
+def test_full_basis_keeps_certified_solutions(cuda):
+    """With max_basis=2 the sweep must still certify every stored result: trial results through
+    residuals(), GMRES results kept outside the full basis through their own FP64 residual."""
+    from emerge_amd.fdsolve import GPUSolver
+    from emerge_amd.sparam import PortSampler
+    from emerge_amd.sweep import BasisSweep, StoredSolution
+    mesh, phys = _lumped_box()
+    prob = phys.problem()
+    freqs = list(np.linspace(2e9, 6e9, 9))
+    tol = 1e-8
+    bs = BasisSweep(prob, GPUSolver(prob, precision=64), PortSampler(prob), tol=tol, max_basis=2)
+    out = bs.run(freqs, "gmres")
+    assert bs.m == 2
+    mask = prob.mask.cpu().numpy().astype(bool)
+    stored = 0
+    for fi, f in enumerate(freqs):
+        for pi in range(len(prob.ports)):
+            y, rec = out[fi][pi]
+            assert rec.converged
+            if isinstance(y, StoredSolution):
+                stored += 1
+                K = prob.to_scipy(prob.system(f)).tocsr()
+                x = y.x.cpu().numpy()
+                b = prob.rhs(prob.ports[pi], f).cpu().numpy()
+                r = b - mask * (K @ (mask * x))
+                rel = np.linalg.norm(r[mask]) / np.linalg.norm(b[mask])
+                assert rel <= tol, (f, pi, rel)
+            else:
+                rel = bs.residuals([(f, pi)], [y])[0]
+                assert rel <= tol * 1.0001, (f, pi, rel)
+    assert stored > 0
