@@ -6,8 +6,9 @@ Robin / port terms and PEC elimination (fem/physics/edm/assembler.py:63-144,
 place of PARDISO, identical in exact arithmetic), field interpolation
 (fem/mth/tet.py:371-497) and S-parameter extraction
 (fem/physics/edm/emfreq3d.py:607-779, fem/mth/sparam.py:72-139,
-fem/mth/integrals.py:24-70).  Boundary-condition objects come from
-``emerge_amd.bc`` (pure data + the reference's port formulas).
+fem/mth/integrals.py:24-70).  Boundary conditions are restated in
+``oracle/ports.py``: the caller's boundary-condition objects contribute their
+data only (kind, tags, frame, dimensions), never their formulas.
 """
 from __future__ import annotations
 
@@ -18,6 +19,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from . import oracle as O
+from . import ports as OP
 
 C0 = 299792458.0
 
@@ -81,7 +83,10 @@ class OracleProblem:
         self.om = O.OracleMesh(mesh.nodes, mesh.tets, order_seed=order_seed)
         self.field = O.OracleNedelec2(self.om)
         self.er, self.ur = er, ur
-        self.bcs = bcs
+        self.bcs = [OP.from_bc(b) for b in bcs]
+        for bc in self.bcs:
+            if bc.kind == "lumped":
+                bc.set_voltage_line(self.om.nodes, np.unique(self.om.tris[:, self.tri_ids(bc.tags)]))
         if E is None:
             E, B = O.tet_mass_stiffness_matrices(self.field, er, ur)
         self.E, self.B = E, B
@@ -96,34 +101,32 @@ class OracleProblem:
     def assemble_freq_matrix(self, frequency):
         k0 = 2 * np.pi * frequency / C0
         K = (self.E - self.B * (k0 ** 2)).tocsr()
-        from emerge_amd.bc import PEC, RobinBC, PortBC
         pec_ids = []
         for bc in self.bcs:
-            if isinstance(bc, PEC):
+            if bc.kind == "pec":
                 tri = self.tri_ids(bc.tags)
                 for e in self.om.tri_to_edge[:, tri].ravel():
                     pec_ids.extend(self.field.edge_to_field[:, e])
                 for t in tri:
                     pec_ids.extend(self.field.tri_to_field[:, t])
         b = np.zeros(self.N, dtype=np.complex128)
-        port_vectors = {bc.port_number: np.zeros(self.N, dtype=np.complex128) for bc in self.bcs if isinstance(bc, PortBC)}
+        port_vectors = {bc.port_number: np.zeros(self.N, dtype=np.complex128) for bc in self.bcs if bc.is_port}
         dpts = gaus_quad_tri4()
         for bc in self.bcs:
-            if not isinstance(bc, RobinBC):
+            if not bc.robin:
                 continue
             tri = self.tri_ids(bc.tags)
-            gamma = bc.get_gamma(k0)
-            if bc._include_force:
+            gamma = bc.gamma(k0)
+            if bc.has_source:
                 # assembler.py:100-123
-                nodes = self.om.nodes
-                vl = bc.get_inv_basis() @ (nodes - bc.cs.origin[:, None])
+                vl = bc.local_vertices(self.om.nodes)
                 blocks = []
                 bvec = np.zeros(self.N, dtype=np.complex128)
                 for itri in tri:
                     vid = self.om.tris[:, itri]
                     x = vl[0, vid] @ dpts[1:]
                     y = vl[1, vid] @ dpts[1:]
-                    U = bc.get_Uinc(x, y, k0)
+                    U = bc.uinc_local(x, y, k0)
                     Bs, bs = _tri_stiff_force(vl[:, vid], gamma, U, dpts)
                     blocks.append(Bs)
                     np.add.at(bvec, self.field.tri_to_field[:, itri], bs)
@@ -132,8 +135,7 @@ class OracleProblem:
                 # assembler.py:125-144
                 blocks = [_tri_stiff(self.om.nodes[:, self.om.tris[:, t]], self.om.edge_lengths[self.om.tri_to_edge[:, t]], gamma)
                           for t in tri]
-            if bc._include_stiff:
-                K = K + _tri_csr(self.field, tri, blocks, self.N)
+            K = K + _tri_csr(self.field, tri, blocks, self.N)
         pec = np.unique(np.asarray(pec_ids, dtype=np.int64))
         solve_ids = np.setdiff1d(np.arange(self.N), pec)
         return K.tocsr(), b, solve_ids, port_vectors
@@ -213,31 +215,29 @@ class OracleProblem:
 
     # fem/physics/edm/emfreq3d.py:734-779
     def s_data(self, bc, x, k0):
-        if bc.v_integration:
-            ln = bc.vintline
-            pts = np.array(ln.cmid)
+        if bc.kind == "lumped":
+            pts, dl = bc.line
             Ex, Ey, Ez = self.interpolate(x, pts, self._candidate_tets(pts))
-            V = np.sum(Ex * ln.dxs + Ey * ln.dys + Ez * ln.dzs)
+            V = np.sum(Ex * dl[0] + Ey * dl[1] + Ez * dl[2])
             if bc.active:
-                a, b = bc.voltage, V - bc.voltage
+                a, b = bc.voltage(), V - bc.voltage()
             else:
                 a, b = 0, V
             return np.sqrt(b ** 2 / (2 * bc.Z0)), np.sqrt(a ** 2 / (2 * bc.Z0))
         tri = self.tri_ids(bc.tags)
         tv = self.om.tris[:, tri]
-        src_tri = self.src.get_triangles(bc.tags)
-        tets_src = self.src.tri_to_tet[0, src_tri]
-        er = self.er[:, :, tets_src]
-        ur = self.ur[:, :, tets_src]
+        tets = self.om.tri_to_tet[0, tri]
+        er = self.er[:, :, tets]
+        ur = self.ur[:, :, tets]
         const = 1 / np.sqrt((ur[0, 0] + ur[1, 1] + ur[2, 2]) / (er[0, 0] + er[1, 1] + er[2, 2]))   # TEM
         dpts = gaus_quad_tri4()
         nodes = self.om.nodes
         P = np.einsum("cvt,vq->cqt", nodes[:, tv], dpts[1:])          # (3, nq, ntri)
         pts = P.reshape(3, -1)
         Ef = self.interpolate(x, pts, self._candidate_tets(pts)).reshape(3, dpts.shape[1], -1)
-        Em = bc.port_mode_3d_global(pts[0], pts[1], pts[2], k0).reshape(3, dpts.shape[1], -1)
+        Em = bc.mode_global(pts[0], pts[1], pts[2]).reshape(3, dpts.shape[1], -1)
         Q = 1 if bc.active else 0
-        Zm = bc.Zmode(k0)
+        Zm = bc.zmode(k0)
         A = self.om.areas[tri] if hasattr(self.om, "areas") else None
         fp = np.sum((Ef - Q * Em) * np.conj(Em), axis=0) / (2 * Zm)    # (nq, ntri)
         mp = np.sum(Em * np.conj(Em), axis=0) / (2 * Zm)
@@ -249,10 +249,9 @@ class OracleProblem:
     def _candidate_tets(self, pts):
         """Tets touching any vertex of the port faces (the reference's all_port_tets)."""
         if not hasattr(self, "_port_tets"):
-            from emerge_amd.bc import PortBC
             verts = set()
             for bc in self.bcs:
-                if isinstance(bc, PortBC):
+                if bc.is_port:
                     verts.update(self.om.tris[:, self.tri_ids(bc.tags)].ravel().tolist())
             mask = np.isin(self.om.tets, np.array(sorted(verts))).any(axis=0)
             self._port_tets = np.flatnonzero(mask)
@@ -260,8 +259,7 @@ class OracleProblem:
 
     # fem/physics/edm/emfreq3d.py:607-732
     def frequency_domain(self, frequencies):
-        from emerge_amd.bc import PortBC
-        ports = [bc for bc in self.bcs if isinstance(bc, PortBC)]
+        ports = [bc for bc in self.bcs if bc.is_port]
         nums = [p.port_number for p in ports]
         S = np.zeros((len(frequencies), len(ports), len(ports)), dtype=np.complex128)
         sols = []

@@ -100,3 +100,84 @@ def test_oracle_numbering_invariance(wg_small):
     other = fd.OracleProblem(prob.src, prob.er, prob.ur, prob.bcs, order_seed=123)
     S2, _ = other.frequency_domain([9e9])
     assert np.abs(S1 - S2).max() < 1e-9
+
+
+# ---------------------------------------------------------------- port physics of the oracle (oracle/ports.py)
+from oracle import ports as OP  # noqa: E402
+
+
+def test_oracle_port_constants():
+    """Literal values of the reference's port formulas (fem/bc.py:549-556, 571-575, 684-713)."""
+    fr = OP.Frame((1, 0, 0), (0, 1, 0), (0, 0, 1), (A_WG / 2, B_WG / 2, 0))
+    wg = OP.OraclePort("waveguide", [5], 1, fr, dims=(A_WG, B_WG))
+    k0 = 2 * np.pi * 10e9 / C0
+    assert abs(wg.beta(k0) - 158.2382563) < 1e-6
+    assert wg.gamma(k0) == 1j * wg.beta(k0)
+    assert wg.zmode(k0) == 376.730313412
+    e = wg.mode_local(np.array([0.0, A_WG / 2]), np.zeros(2))
+    assert np.isclose(e[1, 0], np.sqrt(4 * 377 / (A_WG * B_WG))) and abs(e[1, 1]) < 1e-12 * e[1, 0]
+    lp = OP.OraclePort("lumped", [1], 1, fr, width=2e-3, height=1e-3, direction=(0, 0, 1), Z0=50)
+    assert np.isclose(lp.voltage(), 10.0)                                     # sqrt(2 P Z0)
+    assert np.isclose(lp.gamma(k0), 1j * k0 * 376.7303 / 100.0)               # sheet impedance 50 * 2 / 1
+    assert np.allclose(lp.uinc_local(np.zeros(2), np.zeros(2), k0)[2], -2j * k0 * 10 / 1e-3 * 376.7303 / 100)
+
+
+def _uniform_field_dofs(prob, E):
+    """DOFs of a uniform field by per-tet least squares on the oracle's basis
+    (fem/mth/tet.py:451-492); NED2 holds constants exactly."""
+    f, om = prob.field, prob.om
+    x = np.zeros(prob.N, dtype=np.complex128)
+    cnt = np.zeros(prob.N)
+    seen = {}
+    rng = np.random.default_rng(0)
+    for t in range(om.n_tets):
+        v = om.nodes[:, om.tets[:, t]]
+        pts = v @ rng.dirichlet(np.ones(4), size=12).T
+        dofs = f.tet_to_field[:, t]
+        Phi = np.zeros((36, 20))
+        for k, d in enumerate(dofs):
+            e = np.zeros(prob.N)
+            e[d] = 1.0
+            Phi[:, k] = prob.interpolate(e, pts, [t]).real.ravel()
+        rhs = np.tile(np.asarray(E, dtype=np.float64)[:, None], (1, 12)).ravel()
+        c = np.linalg.lstsq(Phi, rhs, rcond=None)[0]
+        assert np.abs(Phi @ c - rhs).max() < 1e-9 * np.abs(rhs).max()      # the constant is in the space
+        for d, cd in zip(dofs, c):
+            if d in seen:
+                assert abs(seen[d] - cd) <= 1e-8 * max(1.0, abs(cd))          # shared DOFs agree between tets
+            seen[d] = cd
+        x[dofs] = c
+        cnt[dofs] += 1
+    return x
+
+
+def test_oracle_lumped_voltage_of_uniform_field():
+    """V = int E.dl over the port's voltage line (emfreq3d.py:366-389, 753-768) of a uniform
+    field E0 z equals E0 h; the S-data field term is |V| / sqrt(2 Z0)."""
+    size = (6e-3, 4e-3, 3e-3)
+    mesh = meshgen.structured_box(3, 2, 2, size=size, jitter=0.1, seed=5)
+    cs = CoordinateSystem((0, 1, 0), (0, 0, 1), (1, 0, 0), (0.0, size[1] / 2, size[2] / 2))
+    lp = LumpedPort([meshgen.XMIN], 1, width=size[1], height=size[2], direction=(0, 0, 1), cs=cs, Z0=50)
+    er, ur = O.material_tensors(np.ones(mesh.n_tets, complex))
+    prob = fd.OracleProblem(mesh, er, ur, [lp], order_seed=4)
+    E0 = 3.7
+    x = _uniform_field_dofs(prob, [0.0, 0.0, E0])
+    port = prob.bcs[0]
+    mid, dl = port.line
+    assert np.allclose(dl.sum(axis=1), [0, 0, size[2]]) and np.allclose(mid[:2], [[0], [size[1] / 2]])
+    fp, mp = prob.s_data(port, x, 2 * np.pi * 1e9 / C0)
+    assert abs(abs(fp) - E0 * size[2] / np.sqrt(2 * 50)) <= 1e-12 * E0 * size[2]
+    assert mp == 0
+
+
+def test_oracle_waveguide_mode_power():
+    """sparam_mode_power (fem/mth/sparam.py:72-139) of the TE10 mode over the port face:
+    amp^2 a b / 4 / Z_vac = 377 / Z_vac for unit power (4th-order rule on a fine face mesh)."""
+    length = 10e-3
+    mesh = meshgen.waveguide_wr90(16, 8, 4, seed=0, length=length)
+    cs1 = CoordinateSystem((1, 0, 0), (0, 1, 0), (0, 0, 1), (A_WG / 2, B_WG / 2, 0))
+    p1 = RectangularWaveguide([meshgen.ZMIN], 1, cs=cs1, dims=(A_WG, B_WG))
+    er, ur = O.material_tensors(np.ones(mesh.n_tets, complex))
+    prob = fd.OracleProblem(mesh, er, ur, [p1])
+    _, mp = prob.s_data(prob.bcs[0], np.zeros(prob.N, dtype=np.complex128), 2 * np.pi * 10e9 / C0)
+    assert abs(mp - 377 / 376.730313412) < 2e-5
