@@ -30,6 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # 1,702,889) KiB x 2 + WRITE (425,980 + 8,794,723) KiB x 1, the factors a 2 GiB
 # device-to-device copy in the same runs gives.
 ASSEMBLY_TRAFFIC_BYTES = ((52_925 + 1_702_889) * 2 + 425_980 + 8_794_723) * 1024
+# HBM bytes per launch of the sweep's complex64 SpMV (k_spmv_blk<float,16,false>) on the
+# posts box, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this benchmark
+# (--sweep-kernel-only-style run: --spmv-kernel-only --copy-calibration), scaled like the
+# assembly's; None until measured.
+SPMV_TRAFFIC_BYTES = None
 FP64_PEAK_TFLOPS = 78.6
 # FP64 flop convention per tet for the dense contraction of the closed form
 FLOP_PER_TET = 15449
@@ -111,8 +116,15 @@ def bench_sweep(args, dev):
     if args.sweep_kernel_only:
         out["residual_kernel"] = residual_kernel_roofline(mesh, ports, freqs, args.sweep_tol)
         return out
+    if args.spmv_kernel_only:
+        out["spmv_kernel"] = spmv_kernel_roofline(mesh, ports, freqs[len(freqs) // 2])
+        return out
     runs_wanted = set(args.sweep_runs.split(","))
-    for key, mode, prec, fl in (("basis_mixed", "basis", 32, freqs), ("plain_fp64", "plain", 64, freqs[: args.plain_sample])):
+    # plain FP64 GMRES on a sample across the band: both ends and the middle (then evenly spaced)
+    plain_idx = sorted({int(round(i * (len(freqs) - 1) / max(args.plain_sample - 1, 1)))
+                        for i in range(args.plain_sample)}) if args.plain_sample > 0 else []
+    plain_f = [freqs[i] for i in plain_idx]
+    for key, mode, prec, fl in (("basis_mixed", "basis", 32, freqs), ("plain_fp64", "plain", 64, plain_f)):
         if not fl or mode not in runs_wanted:
             continue
         phys = Electrodynamics3D(mesh)
@@ -137,6 +149,10 @@ def bench_sweep(args, dev):
         if mode == "basis":
             out[key]["basis_size"] = int(data.info.get("basis_size", 0))
             out[key]["poly_degree"] = args.poly_degree
+        else:
+            out[key]["freq_indices"] = plain_idx
+            out[key]["rate_note"] = (f"{len(fl)} of {len(freqs)} points timed (indices {plain_idx}); "
+                                     "freq_pts_per_s is their rate")
         print(json.dumps({key: out[key]}), file=sys.stderr, flush=True)
     if args.learned and "learned" in runs_wanted:
         from emerge_amd.fdsolve import GPUSolver
@@ -184,9 +200,9 @@ def bench_sweep(args, dev):
                                     if "basis_mixed" in runs else None)}
         print(json.dumps({"learned_basis_mixed": out["learned_basis_mixed"]}), file=sys.stderr, flush=True)
     out["residual_kernel"] = residual_kernel_roofline(mesh, ports, freqs, args.sweep_tol)
+    out["spmv_kernel"] = spmv_kernel_roofline(mesh, ports, freqs[len(freqs) // 2])
     if "plain_fp64" in runs and "basis_mixed" in runs:
-        k = runs["plain_fp64"].shape[0]
-        out["max_abs_dS_basis_vs_plain"] = float(np.abs(runs["basis_mixed"][:k] - runs["plain_fp64"]).max())
+        out["max_abs_dS_basis_vs_plain"] = float(np.abs(runs["basis_mixed"][plain_idx] - runs["plain_fp64"]).max())
         out["speedup_basis_vs_plain"] = out["basis_mixed"]["freq_pts_per_s"] / out["plain_fp64"]["freq_pts_per_s"]
         if "learned_basis_mixed" in out:
             out["speedup_learned_vs_plain"] = (out["learned_basis_mixed"]["freq_pts_per_s"]
@@ -321,6 +337,48 @@ def eigen_oracle(sigma: float, k: int) -> dict:
     return {"re": w.real.tolist(), "im": w.imag.tolist()}
 
 
+def spmv_kernel_roofline(mesh, ports, freq, reps: int = 50):
+    """The sweep's dominant kernel, the complex64 entity-blocked SpMV of the
+    mixed-precision GMRES steps (k_spmv_blk<float,16,false>, emg_solver_apply
+    precision 32), timed with HIP events on its stream.  Algorithmic bytes per
+    launch (complex64 CSR, int32 indices, one read of x and one write of y):
+    12 nnz + 20 N; the blocked format itself moves about 9 nnz + 20 N (one
+    column index per four entries)."""
+    import torch
+    from emerge_amd import _lib
+    from emerge_amd.fdsolve import GPUSolver
+    from emerge_amd.physics import Electrodynamics3D
+    phys = Electrodynamics3D(mesh)
+    phys.assign(*ports())
+    prob = phys.problem()
+    solver = GPUSolver(prob, precision=32, preconditioner="jacobi", poly_degree=1)
+    solver.set_matrix(prob.system(freq))
+    n, nnz = prob.N, prob.nnz
+    x = torch.randn(n, dtype=torch.complex64, device=prob.dev) * prob.mask
+    y = torch.empty_like(x)
+    L = _lib.load()
+    call = lambda: _lib.check(L.emg_solver_apply(solver.h, 32, _lib.ptr(x), _lib.ptr(y), _lib.stream_ptr()),
+                              "emg_solver_apply")
+    for _ in range(5):
+        call()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    torch.cuda.synchronize()
+    for i in range(reps):
+        ev[i][0].record(stream)
+        call()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    alg = 12 * nnz + 20 * n
+    fmt = 9 * nnz + 20 * n
+    ach = alg / (ms * 1e-3) / 1e9
+    return {"kernel": "k_spmv_blk<float,16,false>", "rows": n, "nnz": nnz, "kernel_ms": ms, "alg_bytes": alg,
+            "format_bytes": fmt, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "format_frac": fmt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": SPMV_TRAFFIC_BYTES}
+
+
 def cpu_sweep_child(name: str, freq: float) -> dict:
     """Run cpu_sweep_baseline in a child process that never touches the GPU (MKL's
     threads and allocator then run as they do in a plain host program)."""
@@ -403,9 +461,11 @@ def main():
     ap.add_argument("--sweep", default="posts", choices=["none", "wr90", "posts"],
                     help="frequency-sweep workload reported beside the assembly figure")
     ap.add_argument("--sweep-freqs", type=int, default=201)
-    ap.add_argument("--plain-sample", type=int, default=2, help="frequencies solved by plain FP64 GMRES")
+    ap.add_argument("--plain-sample", type=int, default=3,
+                    help="frequencies solved by plain FP64 GMRES (band ends and middle first)")
     ap.add_argument("--sweep-tol", type=float, default=1e-8)
-    ap.add_argument("--sweep-cpu", action="store_true", help="time the host-core reference on one sweep frequency")
+    ap.add_argument("--no-sweep-cpu", dest="sweep_cpu", action="store_false",
+                    help="skip timing the host-core reference on one sweep frequency")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip training / using the learned smoother in the sweep")
     ap.add_argument("--learned-steps", type=int, default=60)
@@ -413,7 +473,7 @@ def main():
                     help="degree of the GMRES polynomial around the smoother in the basis sweeps (1 = off)")
     ap.add_argument("--sweep-runs", default="basis,plain,learned",
                     help="comma-separated sweep variants to run: basis, plain, learned")
-    ap.add_argument("--eigen", action="store_true", help="also run the cavity eigenmode workload")
+    ap.add_argument("--no-eigen", dest="eigen", action="store_false", help="skip the cavity eigenmode workload")
     ap.add_argument("--eigen-modes", type=int, default=10)
     ap.add_argument("--eigen-oracle", action="store_true", help="compare the eigenvalues with the CPU oracle (slow)")
     ap.add_argument("--eigen-oracle-child", action="store_true", help=argparse.SUPPRESS)
@@ -421,6 +481,8 @@ def main():
     ap.add_argument("--eigen-k", type=int, default=10, help=argparse.SUPPRESS)
     ap.add_argument("--sweep-kernel-only", action="store_true",
                     help="measure only the batched residual kernel of the sweep (for counter runs)")
+    ap.add_argument("--spmv-kernel-only", action="store_true",
+                    help="measure only the sweep's complex64 SpMV kernel (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device-to-device copy (reference for PMC byte counters)")
     ap.add_argument("--cpu-sweep-child", default=None, help=argparse.SUPPRESS)

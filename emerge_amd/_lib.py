@@ -31,6 +31,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_solver_set_gradient": (ctypes.c_int, [P, I64, P, P, P, P, P, P, P]),
     "emg_solver_solve": (ctypes.c_int, [P, P, P, F64, ctypes.c_int, ctypes.c_int, P, P]),
     "emg_solver_destroy": (None, [P]),
+    "emg_solver_apply": (ctypes.c_int, [P, ctypes.c_int, P, P, P]),
     "emg_affine_combine": (ctypes.c_int, [I64, P, P, F64, P, P]),
     "emg_csr_find": (ctypes.c_int, [I64, P, P, P, P, P, P]),
     "emg_scatter_add": (ctypes.c_int, [I64, P, P, F64, F64, P, P]),
@@ -55,6 +56,10 @@ SIGNATURES: dict[str, tuple] = {
     "emg_gnn_weights": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P]),
     "emg_gnn_weight_grad": (ctypes.c_int, [I64, ctypes.c_int, I64, I64, P, P, P, P, P]),
     "emg_sweep_residual": (ctypes.c_int, [I64, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, P, P, P, P, P]),
+    "emg_csr_permute": (ctypes.c_int, [I64, P, P, P, P, P, P, P, P]),
+    "emg_csr_map": (ctypes.c_int, [I64, P, P, P, P, P, P, P]),
+    "emg_gather_c128": (ctypes.c_int, [I64, P, I64, P, P, P]),
+    "emg_scatter_c128": (ctypes.c_int, [I64, P, I64, P, P, P, P]),
     "emg_port_apply": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P, P, P, P]),
     "emg_basis_combine": (ctypes.c_int, [I64, ctypes.c_int, P, ctypes.c_int, P, P, P]),
 }

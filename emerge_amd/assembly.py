@@ -17,6 +17,7 @@ from .mesh import Mesh3D
 
 EXACT_FACE_MASS = 1
 FORCE_TWO_PHASE = 2
+REFERENCE_GEOMETRY = 4
 
 
 def is_isotropic(t: np.ndarray) -> bool:
@@ -98,14 +99,19 @@ class GPUPattern:
             self._handle = None
 
     def assemble(self, er: np.ndarray, ur: np.ndarray, exact_face_mass: bool = False, out=None,
-                 two_phase: bool = False):
+                 two_phase: bool = False, reference_geometry: bool = False):
         """E, B values (nnz,) complex128 on the device for tensors er, ur (3,3,nt).
 
         two_phase selects the per-tet-block isotropic kernels instead of the
-        fused one (same results; used as a fallback and in tests)."""
+        fused one (same results; used as a fallback and in tests).
+        reference_geometry evaluates the vertex gradients and volumes with the
+        reference's absolute-coordinate formulas (fem/mth/optimized.py:343-369)
+        on the mesh's own tet vertex order, reproducing the reference's rounding;
+        isotropic materials only."""
         import torch
         L = _lib.load()
-        flags = (EXACT_FACE_MASS if exact_face_mass else 0) | (FORCE_TWO_PHASE if two_phase else 0)
+        flags = ((EXACT_FACE_MASS if exact_face_mass else 0) | (FORCE_TWO_PHASE if two_phase else 0)
+                 | (REFERENCE_GEOMETRY if reference_geometry else 0))
         if out is None:
             E = torch.empty(self.nnz, dtype=torch.complex128, device=self.device)
             B = torch.empty(self.nnz, dtype=torch.complex128, device=self.device)
@@ -117,6 +123,8 @@ class GPUPattern:
             nus = torch.from_numpy(np.ascontiguousarray(nu[0, 0])).to(self.device)
             self.assemble_iso_device(eps, nus, flags, E, B)
         else:
+            if reference_geometry:
+                raise ValueError("reference_geometry needs isotropic materials")
             mm = torch.from_numpy(np.ascontiguousarray(np.transpose(er, (2, 0, 1)))).to(self.device)
             ms = torch.from_numpy(np.ascontiguousarray(np.transpose(nu, (2, 0, 1)))).to(self.device)
             _lib.check(L.emg_assemble_tensor(self._handle, _lib.ptr(self.nodes), _lib.ptr(self.tets), _lib.ptr(mm),
@@ -127,7 +135,13 @@ class GPUPattern:
     def assemble_iso_device(self, eps, nu, flags, E, B):
         """Isotropic assembly from device-resident per-tet eps_r and 1/mu_r (complex128)."""
         L = _lib.load()
-        _lib.check(L.emg_assemble_iso(self._handle, _lib.ptr(self.nodes), _lib.ptr(self.tets), _lib.ptr(eps),
+        tets = self.tets
+        if flags & REFERENCE_GEOMETRY:
+            if getattr(self, "tets_ref", None) is None:
+                import torch
+                self.tets_ref = torch.from_numpy(np.ascontiguousarray(self.mesh.tets.T.astype(np.int32))).to(self.device)
+            tets = self.tets_ref
+        _lib.check(L.emg_assemble_iso(self._handle, _lib.ptr(self.nodes), _lib.ptr(tets), _lib.ptr(eps),
                                       _lib.ptr(nu), flags, _lib.ptr(E), _lib.ptr(B), _lib.stream_ptr()),
                    "emg_assemble_iso")
 

@@ -883,14 +883,8 @@ __device__ __forceinline__ uint32_t rbyte(const uint32_t *w, int i) { return (w[
 // Fields: H = (grad x grad) products of the 6 edges (0..20), G = grad.grad of
 // the 4 vertices (21..30), the 6 edge lengths (31..36), eps (37, 38), nu (39, 40).
 template <int S>
-__device__ __forceinline__ void fused_tet_gram(const V3 (&P)[4], cplx ev, cplx nv, double *dst) {
-    const V3 e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]), e3 = sub(P[3], P[0]);
-    V3 gr[4];
-    gr[1] = crs(e2, e3);
-    gr[2] = crs(e3, e1);
-    gr[3] = crs(e1, e2);
-    gr[0] = {-(gr[1].x + gr[2].x + gr[3].x), -(gr[1].y + gr[2].y + gr[3].y), -(gr[1].z + gr[2].z + gr[3].z)};
-    const double V6 = fabs(dot(e1, gr[1]));
+__device__ __forceinline__ void fused_tet_gram(const V3 (&P)[4], const V3 (&gr)[4], double V6, cplx ev, cplx nv,
+                                               double *dst) {
     const double fscale = 1.0 / (5040.0 * V6);
     const double dscale = 1.0 / (120.0 * V6 * V6 * V6);
     constexpr int EU[6] = {0, 0, 0, 1, 1, 2}, EV[6] = {1, 2, 3, 2, 3, 3};
@@ -985,18 +979,91 @@ __device__ __forceinline__ void fused_incidence(const uint32_t *rcp, const doubl
 // rest padding): the first step of the numeric assembly, so each tet's geometry
 // is computed once.  48 doubles = 384 bytes = three whole 128-byte lines.
 static constexpr int REC = 48;
+// The reference's vertex-gradient cofactors and volume from absolute coordinates
+// (fem/mth/optimized.py:343-369), term for term and left to right with no fused
+// multiply-adds, so the cancellation they suffer away from the origin is the
+// reference's own.  Vertices in the reference's (the mesh's) tet order.
+__device__ __attribute__((noinline)) void reference_bcd(const V3 (&Q)[4], double *bb, double *cc, double *dd,
+                                                        double *V) {
+#pragma clang fp contract(off)
+    const double x1 = Q[0].x, x2 = Q[1].x, x3 = Q[2].x, x4 = Q[3].x;
+    const double y1 = Q[0].y, y2 = Q[1].y, y3 = Q[2].y, y4 = Q[3].y;
+    const double z1 = Q[0].z, z2 = Q[1].z, z3 = Q[2].z, z4 = Q[3].z;
+    *V = fabs(-x1*y2*z3/6 + x1*y2*z4/6 + x1*y3*z2/6 - x1*y3*z4/6 - x1*y4*z2/6 + x1*y4*z3/6
+              + x2*y1*z3/6 - x2*y1*z4/6 - x2*y3*z1/6 + x2*y3*z4/6 + x2*y4*z1/6 - x2*y4*z3/6
+              - x3*y1*z2/6 + x3*y1*z4/6 + x3*y2*z1/6 - x3*y2*z4/6 - x3*y4*z1/6 + x3*y4*z2/6
+              + x4*y1*z2/6 - x4*y1*z3/6 - x4*y2*z1/6 + x4*y2*z3/6 + x4*y3*z1/6 - x4*y3*z2/6);
+    bb[0] = -y2*z3 + y2*z4 + y3*z2 - y3*z4 - y4*z2 + y4*z3;
+    bb[1] = y1*z3 - y1*z4 - y3*z1 + y3*z4 + y4*z1 - y4*z3;
+    bb[2] = -y1*z2 + y1*z4 + y2*z1 - y2*z4 - y4*z1 + y4*z2;
+    bb[3] = y1*z2 - y1*z3 - y2*z1 + y2*z3 + y3*z1 - y3*z2;
+    cc[0] = x2*z3 - x2*z4 - x3*z2 + x3*z4 + x4*z2 - x4*z3;
+    cc[1] = -x1*z3 + x1*z4 + x3*z1 - x3*z4 - x4*z1 + x4*z3;
+    cc[2] = x1*z2 - x1*z4 - x2*z1 + x2*z4 + x4*z1 - x4*z2;
+    cc[3] = -x1*z2 + x1*z3 + x2*z1 - x2*z3 - x3*z1 + x3*z2;
+    dd[0] = -x2*y3 + x2*y4 + x3*y2 - x3*y4 - x4*y2 + x4*y3;
+    dd[1] = x1*y3 - x1*y4 - x3*y1 + x3*y4 + x4*y1 - x4*y3;
+    dd[2] = -x1*y2 + x1*y4 + x2*y1 - x2*y4 - x4*y1 + x4*y2;
+    dd[3] = x1*y2 - x1*y3 - x2*y1 + x2*y3 + x3*y1 - x3*y2;
+}
+
+// REF_GEOM = false: tets are canonical (vertices sorted by id); the vertex gradients
+// (times 6V) are cross products of edge vectors relative to vertex 0.
+// REF_GEOM = true: tets are in the reference's vertex order; the gradients and volume
+// come from reference_bcd and are re-indexed to the canonical (sorted) order.
+template <bool REF_GEOM>
 __global__ void __launch_bounds__(256) k_tet_gram(int64_t nt, const double *__restrict__ nodes,
                                                   const int32_t *__restrict__ tets, const cplx *__restrict__ eps,
                                                   const cplx *__restrict__ nu, double *__restrict__ rec) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= nt) return;
     const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
-    const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
-    V3 P[4];
+    int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+    V3 P[4], gr[4];
+    double V6;
+    if (REF_GEOM) {
+        V3 Q[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++)
-        P[j] = {nodes[3 * (int64_t)vid[j]], nodes[3 * (int64_t)vid[j] + 1], nodes[3 * (int64_t)vid[j] + 2]};
-    fused_tet_gram<1>(P, eps[t], nu[t], rec + t * REC);
+        for (int j = 0; j < 4; j++)
+            Q[j] = {nodes[3 * (int64_t)vid[j]], nodes[3 * (int64_t)vid[j] + 1], nodes[3 * (int64_t)vid[j] + 2]};
+        double bb[4], cc[4], dd[4], V;
+        reference_bcd(Q, bb, cc, dd, &V);
+        V6 = 6.0 * V;
+        int ord[4] = {0, 1, 2, 3};   // ord[k]: reference slot of the k-th smallest vertex id
+#pragma unroll
+        for (int i = 1; i < 4; i++)
+#pragma unroll
+            for (int j = i; j > 0; j--)
+                if (vid[ord[j]] < vid[ord[j - 1]]) {
+                    const int tmp = ord[j];
+                    ord[j] = ord[j - 1];
+                    ord[j - 1] = tmp;
+                }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int o = ord[0];
+#pragma unroll
+            for (int j = 1; j < 4; j++) o = (k == j) ? ord[j] : o;
+            double b = bb[0], c = cc[0], d = dd[0];
+            V3 q = Q[0];
+#pragma unroll
+            for (int j = 1; j < 4; j++)
+                if (o == j) { b = bb[j]; c = cc[j]; d = dd[j]; q = Q[j]; }
+            gr[k] = {b, c, d};
+            P[k] = q;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            P[j] = {nodes[3 * (int64_t)vid[j]], nodes[3 * (int64_t)vid[j] + 1], nodes[3 * (int64_t)vid[j] + 2]};
+        const V3 e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]), e3 = sub(P[3], P[0]);
+        gr[1] = crs(e2, e3);
+        gr[2] = crs(e3, e1);
+        gr[3] = crs(e1, e2);
+        gr[0] = {-(gr[1].x + gr[2].x + gr[3].x), -(gr[1].y + gr[2].y + gr[3].y), -(gr[1].z + gr[2].z + gr[3].z)};
+        V6 = fabs(dot(e1, gr[1]));
+    }
+    fused_tet_gram<1>(P, gr, V6, eps[t], nu[t], rec + t * REC);
 }
 
 // Numeric assembly, one workgroup of CH lanes per chunk of CH consecutive
@@ -1197,6 +1264,7 @@ template <bool TENSOR>
 static int assemble(const emg_pattern *pat, const double *d_nodes, const int32_t *d_tets, const double *d_m,
                     const double *d_s, int flags, double *d_E, double *d_B, void *stream) {
     EMG_CHECK(pat && d_nodes && d_tets && d_m && d_s && d_E && d_B, "emg_assemble: null argument");
+    EMG_CHECK(!(flags & EMG_REFERENCE_GEOMETRY), "emg_assemble_tensor: EMG_REFERENCE_GEOMETRY is isotropic-only");
     hipStream_t st = (hipStream_t)stream;
     unsigned grid = (unsigned)((pat->nnz_ent + ABLK - 1) / ABLK);
     k_assemble<TENSOR><<<grid, ABLK, 0, st>>>(*pat, d_nodes, d_tets, reinterpret_cast<const cplx *>(d_m),
@@ -1224,11 +1292,16 @@ extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, c
         const cplx *ce = reinterpret_cast<const cplx *>(d_eps), *cn = reinterpret_cast<const cplx *>(d_nu);
         cplx *oE = reinterpret_cast<cplx *>(d_E), *oB = reinterpret_cast<cplx *>(d_B);
         if (!mp->tet_rec) EMG_HIP(hipMalloc(&mp->tet_rec, (size_t)pat->nt * REC * sizeof(double)));
-        k_tet_gram<<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec);
+        if (flags & EMG_REFERENCE_GEOMETRY)
+            k_tet_gram<true><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec);
+        else
+            k_tet_gram<false><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec);
         launch_staged<CHUNK>(pat, (unsigned)grid, st, mp->tet_rec, efm, oE, oB);
         EMG_LAUNCH_CHECK();
         return 0;
     }
+    EMG_CHECK(!(flags & EMG_REFERENCE_GEOMETRY),
+              "emg_assemble_iso: EMG_REFERENCE_GEOMETRY needs the fused path (not EMG_FORCE_TWO_PHASE)");
     if (!mp->tet_blocks) EMG_HIP(hipMalloc(&mp->tet_blocks, (size_t)pat->nt * TET_BLOCK_DOUBLES * sizeof(double)));
     k_tet_blocks<<<emg::grid_for(pat->nt, 64), 64, 0, st>>>(pat->nt, d_nodes, d_tets,
                                                             (flags & EMG_EXACT_FACE_MASS) ? 1 : 0, mp->tet_blocks);

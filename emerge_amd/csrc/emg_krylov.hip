@@ -1429,6 +1429,25 @@ extern "C" int emg_spmv_blocked(int64_t n, int64_t nnz, const int32_t *d_rowptr,
     return 0;
 }
 
+// The product the solver's Arnoldi steps run: y = mask.(K (mask.x)) with the matrix of the last
+// emg_solver_set_matrix, complex128 (precision 64) or the complex64 copy (precision 32, x and y
+// complex64), PEC columns folded into that copy.
+extern "C" int emg_solver_apply(emg_solver *s, int precision, const void *d_x, void *d_y, void *stream) {
+    EMG_CHECK(s != nullptr && d_x && d_y, "emg_solver_apply: null argument");
+    EMG_CHECK(s->vals != nullptr, "emg_solver_apply: no matrix set");
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == 64) {
+        launch_spmv_blk<double>(st, s->n, s->nnz, s->rowptr, s->cols, s->vals, s->mask,
+                                static_cast<const double *>(d_x), nullptr, static_cast<double *>(d_y));
+    } else {
+        EMG_CHECK(precision == 32 && s->precision == 32, "emg_solver_apply: precision 32 needs a precision-32 solver");
+        launch_spmv_blk<float>(st, s->n, s->nnz, s->rowptr, s->cols, s->vals32, s->mask,
+                               static_cast<const float *>(d_x), nullptr, static_cast<float *>(d_y), false);
+    }
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
 static int nblk_for(int64_t n) { return (int)std::min<int64_t>((n + RB - 1) / RB, 1024); }
 
 // d_h[i] = sum_k conj(V_i[k]) w[k], i < nv; V holds nv vectors of n entries back to back.

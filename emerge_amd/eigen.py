@@ -51,6 +51,7 @@ class ShiftInvertArnoldi:
                                 poly_degree=poly_degree)
         self.solver.set_matrix(self.Ks)
         self.V = torch.zeros((ncv + 1, N), dtype=torch.complex128, device=dev)
+        self.restarts = 0
         self.work = torch.empty(((ncv + 8) // 8) * 1024 * 16 + 1024, dtype=torch.float64, device=dev)
         self.h = torch.empty(2 * (ncv + 1), dtype=torch.float64, device=dev)
         self.nrm = torch.empty(4, dtype=torch.float64, device=dev)
@@ -97,35 +98,119 @@ class ShiftInvertArnoldi:
             hsum += hv[0::2] + 1j * hv[1::2]
         return float(self.nrm[0].item()), hsum
 
-    def run(self, nev: int, spurious_rel: float = 1e-3):
+    def _arnoldi_step(self, j, H) -> bool:
+        """V[j+1], H[:j+2, j] from OP V[j]; False on breakdown (invariant subspace)."""
+        p = self.p
+        y = self._y
+        self._spmv(p.B, self.V[j], y)        # y = B v; PEC rows stay 0
+        z, info = self.solver.solve(y, tol=self.inner_tol, maxiter=self.inner_maxiter)
+        self.inner_iters.append(info.iterations)
+        self.inner_resid.append(info.residual)
+        if not info.converged:
+            raise EigenError(f"inner GMRES did not reach {self.inner_tol:g} (residual {info.residual:.2e} after "
+                             f"{info.iterations} iterations)")
+        if self.verbose:
+            import sys
+            print(f"arnoldi: step {j + 1}/{self.ncv}, inner GMRES {info.iterations} iterations, "
+                  f"residual {info.residual:.1e}", file=sys.stderr, flush=True)
+        hn, hcol = self._orth(j + 1, z)
+        H[: j + 1, j] = hcol
+        H[j + 1, j] = hn
+        if hn < 1e-14 * max(1.0, float(np.abs(hcol).max())):
+            return False
+        self._comb(z[None, :], np.array([1.0 / hn], dtype=np.complex128), self.V[j + 1])
+        return True
+
+    def _wanted(self, theta, nkeep, spurious_rel):
+        """Boolean mask of the nkeep Ritz values theta of OP with the largest |theta|
+        whose eigenvalue sigma + 1/theta is not a gradient (lambda ~ 0) mode."""
+        with np.errstate(divide="ignore", invalid="ignore"):
+            lam = self.sigma + 1.0 / theta
+        ok = np.isfinite(lam) & (np.abs(lam) > spurious_rel * abs(self.sigma))
+        order = [i for i in np.argsort(-np.abs(theta)) if ok[i]][:nkeep]
+        mask = np.zeros(theta.shape[0], dtype=bool)
+        mask[order] = True
+        return mask
+
+    def run(self, nev: int, spurious_rel: float = 1e-3, tol: float = 1e-10, max_restarts: int = 30,
+            resid_tol: float = 1e-8):
+        """The nev modes nearest sigma.  Krylov-Schur restarted Arnoldi on OP =
+        (E - sigma B)^{-1} B: after each pass over ncv vectors the Schur form of
+        the projected OP is reordered so the wanted Ritz values (largest |theta|,
+        gradient modes excluded) lead, and the Krylov relation is truncated to
+        them; converged Ritz values stay in the kept block (locked in place) while
+        the rest keep improving.  Stops when nev Ritz pairs have residual estimates
+        |h_{m+1,m} e_m^T s| <= tol |theta|, then an FP64 Rayleigh-Ritz step on
+        span(V) gives the eigenvalues; every returned pair must have a true
+        relative residual ||E x - lambda B x|| / ||E x|| <= resid_tol, else
+        EigenError (no silent partial answer)."""
+        import scipy.linalg as sla_
         torch = self.torch
         p = self.p
         m = self.ncv
+        if nev < 1 or 2 * nev + 4 > m:
+            raise ValueError(f"need 1 <= nev and ncv >= 2 nev + 4, got nev={nev}, ncv={m}")
         rng = np.random.default_rng(self.seed)
         v0 = rng.standard_normal(p.N) * p.mask.cpu().numpy()
         v = torch.from_numpy(v0.astype(np.complex128)).to(p.dev)
         self._comb(v[None, :], np.array([1.0 / self._norm(v)], dtype=np.complex128), self.V[0])
+        self._y = torch.empty(p.N, dtype=torch.complex128, device=p.dev)
         H = np.zeros((m + 1, m), dtype=np.complex128)
-        y = torch.empty(p.N, dtype=torch.complex128, device=p.dev)
-        k = m
-        for j in range(m):
-            self._spmv(p.B, self.V[j], y)        # y = B v; PEC rows stay 0
-            z, info = self.solver.solve(y, tol=self.inner_tol, maxiter=self.inner_maxiter)
-            self.inner_iters.append(info.iterations)
-            self.inner_resid.append(info.residual)
+        k = 0                      # OP V[:k] = V[:k+1] H[:k+1, :k] holds
+        nkeep = min(m - 2, max(nev + 2, (nev + m) // 2))
+        self.restarts = 0
+        size = m
+        for it in range(max_restarts + 1):
+            size = m
+            for j in range(k, m):
+                if not self._arnoldi_step(j, H):
+                    size = j + 1                       # invariant subspace: Ritz values are exact
+                    break
+            T, Q = sla_.schur(H[:size, :size], output="complex")
+            theta = np.diag(T).copy()
+            # residual estimates of the Ritz pairs (eigenvectors s of T, Ritz vectors V Q s)
+            w, S = np.linalg.eig(T)
+            est = np.abs(H[size, size - 1]) * np.abs((Q[size - 1, :] @ S)) if size == m else np.zeros(size)
+            want = self._wanted(w, nev, spurious_rel)
+            nconv = int(np.sum(want & (est <= tol * np.abs(w))))
             if self.verbose:
                 import sys
-                print(f"arnoldi: step {j + 1}/{m}, inner GMRES {info.iterations} iterations, residual {info.residual:.1e}",
-                      file=sys.stderr, flush=True)
-            hn, hcol = self._orth(j + 1, z)
-            H[: j + 1, j] = hcol
-            H[j + 1, j] = hn
-            if hn < 1e-14:
-                k = j + 1
+                print(f"arnoldi: pass {it + 1}: {nconv}/{nev} Ritz pairs converged", file=sys.stderr, flush=True)
+            if nconv >= nev or size < m or it == max_restarts:
                 break
-            self._comb(z[None, :], np.array([1.0 / hn], dtype=np.complex128), self.V[j + 1])
-        Vk = self.V[:k]
-        # FP64 Rayleigh-Ritz on span(V): (V^H E V) c = lambda (V^H B V) c
+            # Krylov-Schur restart: the nkeep wanted Schur vectors first, truncate to them
+            sel = self._wanted(theta, nkeep, spurious_rel)
+            keep_vals = theta[sel]
+            T2, Q2, sdim = sla_.schur(H[:m, :m], output="complex",
+                                      sort=lambda z: bool(np.any(np.isclose(z, keep_vals, rtol=1e-12, atol=0.0))))
+            kk = int(sdim)
+            if kk < 1:
+                break
+            Vnew = torch.empty((kk, p.N), dtype=torch.complex128, device=p.dev)
+            for i in range(kk):
+                self._comb(self.V[:m], Q2[:, i].copy(), Vnew[i])
+            self.V[:kk].copy_(Vnew)
+            self.V[kk].copy_(self.V[m])
+            del Vnew
+            Hn = np.zeros_like(H)
+            Hn[:kk, :kk] = T2[:kk, :kk]
+            Hn[kk, :kk] = H[m, m - 1] * Q2[m - 1, :kk]
+            H = Hn
+            k = kk
+            self.restarts += 1
+        # the wanted Schur vectors of the projected OP span the converged invariant subspace;
+        # Rayleigh-Ritz on all of span(V) would admit spurious interior Ritz values near sigma
+        T, Q = sla_.schur(H[:size, :size], output="complex")
+        keep_vals = np.diag(T)[self._wanted(np.diag(T), nev, spurious_rel)]
+        _, Qw, sdim = sla_.schur(H[:size, :size], output="complex",
+                                 sort=lambda z: bool(np.any(np.isclose(z, keep_vals, rtol=1e-12, atol=0.0))))
+        k = int(sdim)
+        if k < nev:
+            raise EigenError(f"only {k} non-gradient Ritz values for {nev} requested modes (Krylov space of {size})")
+        Vk = torch.empty((k, p.N), dtype=torch.complex128, device=p.dev)
+        for i in range(k):
+            self._comb(self.V[:size], Qw[:, i].copy(), Vk[i])
+        # FP64 Rayleigh-Ritz on that subspace: (W^H E W) c = lambda (W^H B W) c
         EV = torch.empty_like(Vk)
         BV = torch.empty_like(Vk)
         for i in range(k):
@@ -139,6 +224,7 @@ class ShiftInvertArnoldi:
                                             _lib.stream_ptr()), "emg_mdotc")
                 hv = self.h[: 2 * k].cpu().numpy()
                 out[:, i] = hv[0::2] + 1j * hv[1::2]
+        del EV, BV
         lam, C = sla.eig(Ap, Bp)
         keep = np.isfinite(lam) & (np.abs(lam) > spurious_rel * abs(self.sigma))
         lam, C = lam[keep], C[:, keep]
@@ -155,7 +241,18 @@ class ShiftInvertArnoldi:
             self._spmv(p.B, X[i], bx)
             ne = self._norm(ex)
             resid.append(self._axpy_norm(ex, bx, complex(lam[i])) / ne)
-        return lam, X, np.array(resid)
+        resid = np.array(resid)
+        good = int(np.sum(resid <= resid_tol))
+        if lam.shape[0] < nev or good < nev:
+            raise EigenError(f"{good} of {nev} requested modes reached relative residual {resid_tol:g} after "
+                             f"{self.restarts} restarts (residuals {np.array2string(resid, precision=2)}); "
+                             "raise ncv or max_restarts")
+        order = np.argsort(lam.real)
+        return lam[order], X[torch.from_numpy(order).to(p.dev)], resid[order]
+
+
+class EigenError(RuntimeError):
+    """The eigen solve could not deliver the requested modes to the stated residual."""
 
 
 def eigen_frequencies(lam) -> np.ndarray:

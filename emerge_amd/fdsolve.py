@@ -94,14 +94,15 @@ class FrequencyProblem:
     """E, B, Robin terms and the PEC mask of one mesh on one GPU."""
 
     def __init__(self, mesh: Mesh3D, er: np.ndarray, ur: np.ndarray, bcs, exact_face_mass: bool = False,
-                 device: str = "cuda", pattern: GPUPattern | None = None):
+                 device: str = "cuda", pattern: GPUPattern | None = None, reference_geometry: bool = False):
         import torch
         self.torch = torch
         self.mesh = mesh
         self.bcs = list(bcs)
         self.dev = torch.device(device)
         self.pat = GPUPattern(mesh, device=device) if pattern is None else pattern
-        self.E, self.B = self.pat.assemble(er, ur, exact_face_mass=exact_face_mass)
+        self.E, self.B = self.pat.assemble(er, ur, exact_face_mass=exact_face_mass,
+                                           reference_geometry=reference_geometry)
         self.er, self.ur = er, ur
         self.N, self.nnz = self.pat.n, self.pat.nnz
         mask = np.ones(self.N, dtype=np.uint8)
@@ -168,12 +169,14 @@ class FrequencyProblem:
                                          _lib.ptr(K), _lib.stream_ptr()), "emg_scatter_add")
         return K
 
-    def rhs(self, port: PortBC, freq: float):
-        """Port source vector b_p(f) on the device (masked to free DOFs)."""
+    def rhs(self, port: PortBC, freq: float, masked: bool = True):
+        """Port source vector b_p(f) on the device (masked to free DOFs unless masked=False,
+        which gives the reference's port_vectors entry, assembler.py:378)."""
         k0 = 2 * np.pi * freq / C0
         for rt in self.robin:
             if rt.bc is port:
-                return rt.q * complex(port.force_scale(k0)) * self.mask
+                q = rt.q * complex(port.force_scale(k0))
+                return q * self.mask if masked else q
         raise KeyError(f"port {port.port_number} has no source term")
 
     def to_scipy(self, vals):

@@ -71,6 +71,7 @@ class Electrodynamics3D:
         self.materials: dict[int, Material] = dict(materials or {})
         self.solveroutine = GPUSolveSettings()
         self.exact_face_mass = False
+        self.reference_geometry = False     # reference's absolute-coordinate element geometry (its rounding)
         self.data: EMSimData | None = None
         self._problem = None
         self._problem_key = None
@@ -129,7 +130,7 @@ class Electrodynamics3D:
         """The device-resident frequency problem (built once per mesh/material/BC set)."""
         from .fdsolve import FrequencyProblem
         key = (id(self.mesh), tuple(sorted((k, v.er, v.ur, v.tand) for k, v in self.materials.items())),
-               tuple(id(b) for b in self.boundary_conditions), self.exact_face_mass)
+               tuple(id(b) for b in self.boundary_conditions), self.exact_face_mass, self.reference_geometry)
         if self._problem is None or self._problem_key != key:
             er, ur = self.material_tensors()
             for p in self.ports():
@@ -137,7 +138,8 @@ class Electrodynamics3D:
                     from .sparam import lumped_port_line
                     lumped_port_line(self.mesh, p)
             self._problem = FrequencyProblem(self.mesh, er, ur, self.boundary_conditions,
-                                             exact_face_mass=self.exact_face_mass)
+                                             exact_face_mass=self.exact_face_mass,
+                                             reference_geometry=self.reference_geometry)
             self._problem_key = key
         return self._problem
 
@@ -226,21 +228,34 @@ class Electrodynamics3D:
         self.data = data
         return data
 
-    def eigenmode(self, num_sols: int = 6, target_frequency: float | None = None, ncv: int | None = None,
-                  inner_tol: float = 1e-12, spurious_rel: float = 1e-3) -> EMSimData:
+    def eigenmode(self, mesh: Mesh3D | None = None, solver=None, num_sols: int = 6,
+                  target_frequency: float | None = None, ncv: int | None = None, inner_tol: float = 1e-12,
+                  spurious_rel: float = 1e-3) -> EMSimData:
         """Modes of E x = k^2 B x nearest k(target)^2 with PEC walls (emfreq3d.py:815-858).
 
+        Same leading arguments as the reference's (mesh, solver, num_sols); the
+        target is the first frequency of set_frequency (emfreq3d.py:837) unless
+        target_frequency is given.  mesh: a mesh to solve on instead of the
+        physics' own; solver: a GPUSolveRoutine whose preconditioner and
+        polynomial settings the inner solves take (None: solveroutine's).
         Returns an EMSimData with one dataset per mode: freq (Hz), k0, mode,
-        residual, and the mode field as _mode_field."""
+        residual, and the mode field as _mode_field; raises EigenError if fewer
+        than num_sols modes reach the residual target."""
         from .eigen import ShiftInvertArnoldi, eigen_frequencies
+        if mesh is not None and mesh is not self.mesh:
+            self.mesh = mesh
+            self._problem = None
+            if not self._bc_initialized:
+                self._initialize_bcs()
+        st = solver if solver is not None else self.solveroutine
         f_t = target_frequency if target_frequency is not None else (self.frequencies[0] if self.frequencies else None)
         if f_t is None:
             raise SimulationError("eigenmode needs a target frequency")
         sigma = (2 * np.pi * f_t / C0) ** 2                                      # emfreq3d.py:838-839
         prob = self.problem()
         arn = ShiftInvertArnoldi(prob, sigma, ncv=ncv or max(4 * num_sols + 20, 24), inner_tol=inner_tol,
-                                 preconditioner=self.solveroutine.preconditioner, verbose=self.solveroutine.verbose,
-                                 poly_degree=self.solveroutine.poly_degree)
+                                 preconditioner=st.preconditioner, verbose=getattr(st, "verbose", False),
+                                 poly_degree=st.poly_degree)
         lam, X, resid = arn.run(num_sols, spurious_rel=spurious_rel)
         data = EMSimData()
         for i in range(lam.shape[0]):
@@ -251,5 +266,6 @@ class Electrodynamics3D:
             ds._mode_field = X[i].cpu().numpy()
         data.info["inner_iterations"] = list(arn.inner_iters)
         data.info["sigma"] = sigma
+        data.info["restarts"] = arn.restarts
         self.data = data
         return data

@@ -55,6 +55,13 @@ void emg_pattern_destroy(emg_pattern *pat);
                                  the reference's coefficient (fem/mth/tet.py:1036) */
 #define EMG_FORCE_TWO_PHASE 2 /* isotropic assembly through per-tet element blocks in
                                  HBM instead of the fused LDS kernel (same results) */
+#define EMG_REFERENCE_GEOMETRY 4 /* isotropic fused path only: d_tets hold each tet in the
+                                 reference's vertex order, and the vertex gradients and volume
+                                 come from the reference's absolute-coordinate cofactor formulas
+                                 (fem/mth/optimized.py:343-369, no fused multiply-adds), so the
+                                 matrices carry the reference's own rounding; the default uses
+                                 edge vectors relative to one vertex (more accurate away from
+                                 the origin) */
 
 /* Numeric assembly of the curl-curl matrix E (Dmat, with 1/mu) and the mass
  * matrix B (Fmat, with eps) into the pattern's CSR value arrays, complex128.
@@ -129,6 +136,11 @@ int emg_solver_set_poly(emg_solver *s, int degree, const double *h_coef);
  * the (d + 1) x d Hessenberg matrix column-major as complex128.  Uses the
  * solver's Krylov storage: not during a solve.  Synchronises `stream`.       */
 int emg_solver_arnoldi_probe(emg_solver *s, int d, const double *d_v0, double *h_out, void *stream);
+/* y = mask.(K (mask.x)) as the solver's Arnoldi steps form it, with the matrix of
+ * the last emg_solver_set_matrix: precision 64 complex128, precision 32 the
+ * solver's complex64 copy (x, y complex64).  No counterpart in the reference
+ * (scipy's CSR product inside its solvers); exported to time the product. */
+int emg_solver_apply(emg_solver *s, int precision, const void *d_x, void *d_y, void *stream);
 /* d_x: initial guess in, solution out (n complex128). Synchronises `stream`. */
 int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, double tol, int maxiter, int max_refine,
                      emg_solve_info *info, void *stream);
@@ -215,6 +227,31 @@ int emg_port_apply(int64_t nprow, int nvec, const int32_t *d_ptr, const int32_t 
 /* X[row, j] = sum_i V[i, row] Y[j, i]: d_V (m, n) c128 basis, d_Y (nvec, m) c128,
  * nvec * m <= 2048.                                                            */
 int emg_basis_combine(int64_t n, int m, const double *d_V, int nvec, const double *d_Y, double *d_X, void *stream);
+
+/* ------------------------------------------------------------------ numbering maps
+ * The reference numbers edges and faces in Python set order
+ * (fem/mesh3d.py:252-271); these move matrices and vectors between a caller's
+ * numbering and the library's on the GPU, so tet_mass_stiffness_matrices
+ * (fem/physics/edm/optimized_assembly.py:43-64), assemble_freq_matrix
+ * (fem/physics/edm/assembler.py:312-388) and SolveRoutine.solve
+ * (fem/solver.py:405-469) keep the caller's numbering.  d_p (n int32): the
+ * library index of caller DOF i; d_pinv: its inverse.
+ *
+ * emg_csr_permute: caller-order CSR of a matrix on the library CSR (d_rowptr,
+ *   d_cols, n rows): d_rowptr_out (n+1), d_cols_out (nnz, sorted per row) and
+ *   d_map (nnz): the library entry of every caller entry.  Synchronises.
+ * emg_csr_map: d_pos (nnz of the caller CSR) = library position of every entry
+ *   of a caller-order CSR (d_rowptr_c, d_cols_c), -1 where the pattern lacks it.
+ * emg_gather_c128:  d_dst[k] = d_src[d_idx[k]] (0 where d_idx[k] is outside [0, n_src)).
+ * emg_scatter_c128: d_dst[d_idx[k]] = d_src[k] for d_idx[k] in [0, n_dst); nonzero
+ *   values with no target are counted into *d_dropped (uint64, may be NULL).   */
+int emg_csr_permute(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const int32_t *d_p,
+                    const int32_t *d_pinv, int32_t *d_rowptr_out, int32_t *d_cols_out, int32_t *d_map, void *stream);
+int emg_csr_map(int64_t n, const int32_t *d_rowptr_c, const int32_t *d_cols_c, const int32_t *d_p,
+                const int32_t *d_rowptr, const int32_t *d_cols, int32_t *d_pos, void *stream);
+int emg_gather_c128(int64_t n, const int32_t *d_idx, int64_t n_src, const double *d_src, double *d_dst, void *stream);
+int emg_scatter_c128(int64_t n, const int32_t *d_idx, int64_t n_dst, const double *d_src, double *d_dst,
+                     unsigned long long *d_dropped, void *stream);
 
 /* ------------------------------------------------------------------ learned smoother
  * Kernels of the graph network that predicts the smoother weights from mesh

@@ -136,3 +136,18 @@ def test_full_size_assembly_box(cuda):
         assert err < 1e-12
     sub = Eh[rows_g][:, rows_g]
     assert abs(sub - sub.T).max() < 1e-12 * abs(sub).max()
+    # the reference's own arithmetic: the default kernels against the fp64 restatement (reported), and
+    # the reference-geometry mode (absolute-coordinate cofactors, fem/mth/optimized.py:343-369) against it
+    E, B = pat.assemble(er, ur, reference_geometry=True)
+    torch.cuda.synchronize()
+    Er, Br = pat.to_scipy(E), pat.to_scipy(B)
+    del E, B
+    E64o, B64o = O.tet_mass_stiffness_matrices(of, er[:, :, sel], ur[:, :, sel])
+    for Mg, Mr, Mo, name in ((Eh, Er, E64o, "E"), (Bh, Br, B64o, "B")):
+        o = Mo[rows_o].tocoo()
+        o2 = csr_matrix((o.data, (o.row, o2m[o.col])), shape=(rows_g.shape[0], pat.n))
+        e_def = rel_fro(Mg[rows_g], o2)
+        e_ref = rel_fro(Mr[rows_g], o2)
+        print(f"{name}: gpu default vs fp64 oracle {e_def:.2e}; gpu reference geometry vs fp64 oracle {e_ref:.2e}",
+              flush=True)
+        assert e_ref < 1e-12

@@ -51,3 +51,47 @@ def test_eigenmodes_match_arpack(cuda, block):
                         exact += [(m * np.pi / a) ** 2 + (n * np.pi / b) ** 2 + (p * np.pi / d) ** 2] * (2 if nz == 3 else 1)
         near = sorted(exact, key=lambda v: abs(v - (2 * np.pi * ftarget / C0) ** 2))[:4]
         assert np.allclose(np.sort(lam_g.real), np.sort(near), rtol=0.02)
+
+
+def test_restarted_arnoldi_finds_the_oracle_set(cuda):
+    """Krylov-Schur restarts with a short basis (ncv = 2 nev + 4) return the same sorted
+    set of eigenvalues as ARPACK (a skipped or spurious mode fails the set comparison)."""
+    from emerge_amd.eigen import ShiftInvertArnoldi
+    mesh, phys, size = _cavity(True)
+    ftarget, nev = 12e9, 6
+    sigma = (2 * np.pi * ftarget / C0) ** 2
+    arn = ShiftInvertArnoldi(phys.problem(), sigma, ncv=2 * nev + 4)
+    lam_g, X, res = arn.run(nev)
+    print("restarts", arn.restarts, "residuals", res, flush=True)
+    er, ur = phys.material_tensors()
+    lam_o = fd.oracle_eigenmodes(fd.OracleProblem(mesh, er, ur, phys.boundary_conditions, order_seed=2), sigma, nev)
+    lam_o = np.sort(lam_o.real)
+    assert lam_g.shape[0] == nev and np.all(res <= 1e-8)
+    assert np.all(np.abs(np.sort(lam_g.real) - lam_o) / lam_o < 1e-8), (lam_g, lam_o)
+
+
+def test_eigen_refuses_unmet_residual(cuda):
+    """A residual target the pairs cannot meet raises instead of returning them."""
+    from emerge_amd.eigen import EigenError, ShiftInvertArnoldi
+    mesh, phys, size = _cavity(False)
+    arn = ShiftInvertArnoldi(phys.problem(), (2 * np.pi * 10e9 / C0) ** 2, ncv=12)
+    with pytest.raises(EigenError):
+        arn.run(4, resid_tol=1e-30, max_restarts=0)
+
+
+def test_cavity_workload_modes_match_oracle_fixture(cuda):
+    """Full cavity workload (221,184 tets, eps_r 6 block): the 10 modes nearest 5 GHz equal the
+    oracle's (ARPACK shift-invert with PARDISO, tests/golden/oracle_fixtures.npz) as a sorted set."""
+    import os
+    from emerge_amd import workloads
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_fixtures.npz"))
+    lam_o = np.sort(gold["c4_lambda"].real)
+    phys = workloads.cavity()
+    print(f"cavity: {phys.mesh.n_tets} tets", flush=True)
+    data = phys.eigenmode(num_sols=10, target_frequency=workloads.CAVITY_TARGET_HZ)
+    lam_g = np.sort(np.array([ds.eigenvalue for ds in data.datasets]).real)
+    res = np.array([ds.residual for ds in data.datasets])
+    rel = np.abs(lam_g - lam_o) / lam_o
+    print("gpu", lam_g, "max rel", rel.max(), "residuals", res.max(), flush=True)
+    assert float(gold["c4_sigma"]) == pytest.approx((2 * np.pi * workloads.CAVITY_TARGET_HZ / C0) ** 2, rel=1e-15)
+    assert lam_g.shape == lam_o.shape and np.all(rel < 1e-8)
