@@ -60,6 +60,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_csr_map": (ctypes.c_int, [I64, P, P, P, P, P, P, P]),
     "emg_gather_c128": (ctypes.c_int, [I64, P, I64, P, P, P]),
     "emg_scatter_c128": (ctypes.c_int, [I64, P, I64, P, P, P, P]),
+    "emg_sweep_residual_noalloc": (ctypes.c_int, [I64, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, P, P, P, P]),
     "emg_port_apply": (ctypes.c_int, [I64, ctypes.c_int, P, P, P, P, P, P, P, P]),
     "emg_basis_combine": (ctypes.c_int, [I64, ctypes.c_int, P, ctypes.c_int, P, P, P]),
 }

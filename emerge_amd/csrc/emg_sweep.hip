@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/emerge_amd.h"
@@ -237,6 +238,56 @@ extern "C" int emg_sweep_residual(int64_t n, const int32_t *d_rowptr, const int3
     k_sweep_norms<<<2 * nvec, 256, 0, st>>>(grid, nvec, d_work, d_norms);
     EMG_LAUNCH_CHECK();
     return 0;
+}
+
+// ------------------------------------------------------------ no-allocation variant
+// The residual's per-block partial sums live in a file-scope device array
+// instead of a caller workspace, for callers that cannot allocate (code under
+// stream capture, serving loops).  The host side never uses the array's name
+// as an address: its device address is looked up once per device with
+// hipGetSymbolAddress and cached.  Every block of the residual kernel writes
+// its own partial entries before k_sweep_norms reads them, so the array needs
+// no clearing between calls; calls that overlap on one device race on it.
+constexpr int64_t NOALLOC_SCRATCH_DOUBLES = (int64_t)RES_GRID_MAX * MAX_VEC * 2;   // 1 MiB
+__device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_for_current_device(double **out) {
+    int dev = 0;
+    EMG_HIP(hipGetDevice(&dev));
+    EMG_CHECK(dev >= 0 && dev < MAX_DEVICES, "emg_sweep_residual_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        EMG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        EMG_CHECK(p != nullptr, "emg_sweep_residual_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" int emg_sweep_residual_noalloc(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols,
+                                          const double *d_E, const double *d_B, const uint8_t *d_mask, int nvec,
+                                          const double *d_s, const double *d_X, const double *d_Q,
+                                          const int32_t *d_qidx, const double *d_c, const int32_t *d_prow_of_row,
+                                          const double *d_Zp, double *d_R, double *d_norms, void *stream) {
+    EMG_CHECK(valid_nvec(nvec), "emg_sweep_residual_noalloc: nvec must be 1, 2, 4, 8 or 16");
+    EMG_CHECK(n > 0 && n % 2 == 0, "emg_sweep_residual_noalloc: n must be 2 * entities (an entity pattern)");
+    constexpr int ents_per_block = RES_BLOCK / ROW_LANES;
+    const int64_t grid = std::min<int64_t>(RES_GRID_MAX, (n / 2 + ents_per_block - 1) / ents_per_block);
+    EMG_CHECK(grid * nvec * 2 <= NOALLOC_SCRATCH_DOUBLES,
+              "emg_sweep_residual_noalloc: partial sums exceed the 1 MiB scratch array");
+    double *work = nullptr;
+    const int rc = scratch_for_current_device(&work);
+    if (rc != 0) return rc;
+    return emg_sweep_residual(n, d_rowptr, d_cols, d_E, d_B, d_mask, nvec, d_s, d_X, d_Q, d_qidx, d_c,
+                              d_prow_of_row, d_Zp, d_R, d_norms, work, stream);
 }
 
 extern "C" int emg_port_apply(int64_t nprow, int nvec, const int32_t *d_ptr, const int32_t *d_pcol,

@@ -219,6 +219,18 @@ int emg_sweep_residual(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols
                        const double *d_B, const uint8_t *d_mask, int nvec, const double *d_s, const double *d_X,
                        const double *d_Q, const int32_t *d_qidx, const double *d_c, const int32_t *d_prow_of_row,
                        const double *d_Zp, double *d_R, double *d_norms, double *d_work, void *stream);
+/* The same residual with no workspace argument and no allocation: the
+ * per-block partial sums live in the library's static 1 MiB device array
+ * (one per GPU, held until the process ends).  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP stream
+ * capture, tight serving loops).  Limits as emg_sweep_residual (nvec <= 16);
+ * larger requests are refused.  Calls that overlap on one GPU race on the
+ * array: serialise them (one stream, or events).                             */
+int emg_sweep_residual_noalloc(int64_t n, const int32_t *d_rowptr, const int32_t *d_cols, const double *d_E,
+                               const double *d_B, const uint8_t *d_mask, int nvec, const double *d_s,
+                               const double *d_X, const double *d_Q, const int32_t *d_qidx, const double *d_c,
+                               const int32_t *d_prow_of_row, const double *d_Zp, double *d_R, double *d_norms,
+                               void *stream);
 /* Robin rows: Zp[i, j] = sum_e g[term_e, j] * val_e * X[col_e, j] over the
  * entries d_ptr[i] .. d_ptr[i+1] of port row i (B_p(f) = gamma_p(f) P_p,
  * assembler.py:382-383).  d_g (nterm, nvec) c128.                              */
