@@ -74,23 +74,86 @@ class Line:
 
 
 # ------------------------------------------------------------------ materials
+def _is_scalar(v) -> bool:
+    return isinstance(v, (float, complex, int, np.floating, np.complexfloating, np.integer))
+
+
 @dataclass
 class Material:
-    """fem/material.py:20-106 (scalar er/ur with loss tangent)."""
-    er: float | complex = 1.0
-    ur: float | complex = 1.0
+    """fem/material.py:20-106.
+
+    er, ur: scalars (times the identity) or (3, 3) tensors; tand multiplies
+    er by (1 - j tand).  _fer, _fur: optional callables (x, y, z) -> (3, 3, N)
+    complex tensors at N points (tet centroids), e.g. PML stretches; they take
+    precedence over er / ur, as in the reference."""
+    er: float | complex | np.ndarray = 1.0
+    ur: float | complex | np.ndarray = 1.0
     tand: float = 0.0
+    _fer: object = None
+    _fur: object = None
 
     @property
-    def ermat(self) -> np.ndarray:                                          # fem/material.py:44-48
-        return self.er * (1 - 1j * self.tand) * np.eye(3)
+    def ermat(self) -> np.ndarray:                                          # fem/material.py:44-49
+        if _is_scalar(self.er):
+            return self.er * (1 - 1j * self.tand) * np.eye(3)
+        return np.asarray(self.er) * (1 - 1j * self.tand)
 
     @property
-    def urmat(self) -> np.ndarray:
-        return self.ur * np.eye(3)
+    def urmat(self) -> np.ndarray:                                          # fem/material.py:51-56
+        if _is_scalar(self.ur):
+            return self.ur * np.eye(3)
+        return np.asarray(self.ur)
+
+    @property
+    def fer3d_mat(self):                                                    # fem/material.py:91-96
+        if self._fer is None:
+            return lambda x, y, z: np.repeat(self.ermat[:, :, None], np.shape(x)[0], axis=2)
+        return self._fer
+
+    @property
+    def fur3d_mat(self):                                                    # fem/material.py:98-103
+        if self._fur is None:
+            return lambda x, y, z: np.repeat(self.urmat[:, :, None], np.shape(x)[0], axis=2)
+        return self._fur
+
+    def key(self):
+        """Identity for caching: value fields, or the callables / tensors by object identity."""
+        v = lambda a: a if _is_scalar(a) else id(a)
+        return (v(self.er), v(self.ur), self.tand, id(self._fer) if self._fer is not None else None,
+                id(self._fur) if self._fur is not None else None)
 
 
 AIR = Material()
+
+
+def pml_material(material: Material, center, dims, direction, thickness: float, exponent: float = 3.0,
+                 deltamax: float = 8.0) -> Material:
+    """A perfectly matched layer of `material` on the side `direction` (one of +-1/0 per
+    axis) of a box with centre `center` and size `dims`: diagonal er and ur tensors
+    er * diag(sy sz / sx, sz sx / sy, sx sy / sz) with the stretches
+    s_i = 1 - j (d_i (x_i - c_i - d_i D_i / 2) / thickness)^exponent deltamax on the
+    layered axes and 1 elsewhere (fem/modeling/pmlbox.py:25-92)."""
+    (px, py, pz), (W, D, H), (dx, dy, dz) = center, dims, direction
+
+    def stretch(d, c, size):
+        if d == 0:
+            return lambda x: np.ones(np.shape(x)[0], dtype=np.complex128)
+        return lambda x: 1 - 1j * (d * (x - c - d * size / 2) / thickness) ** exponent * deltamax
+
+    sxf, syf, szf = stretch(dx, px, W), stretch(dy, py, D), stretch(dz, pz, H)
+
+    def tensor(base):
+        def f(x, y, z):
+            sx, sy, sz = sxf(x), syf(y), szf(z)
+            out = np.zeros((3, 3, np.shape(x)[0]), dtype=np.complex128)
+            out[0, 0] = base * sy * sz / sx
+            out[1, 1] = base * sz * sx / sy
+            out[2, 2] = base * sx * sy / sz
+            return out
+        return f
+
+    er = material.er * (1 - 1j * material.tand)
+    return Material(er=material.er, ur=material.ur, tand=material.tand, _fer=tensor(er), _fur=tensor(material.ur))
 
 
 # ------------------------------------------------------------------ boundary conditions

@@ -114,22 +114,25 @@ class Electrodynamics3D:
         self._problem = None
 
     def material_tensors(self):
-        """(3,3,nt) complex eps_r and mu_r per tet (fem/mesh3d.py:358-378)."""
+        """(3,3,nt) complex eps_r and mu_r per tet: each volume tag's material evaluated at its
+        tet centroids through fer3d_mat / fur3d_mat (fem/mesh3d.py:358-378,
+        fem/material.py:91-103), so callable and tensor materials (PML) work as in the reference."""
         nt = self.mesh.n_tets
         er = np.zeros((3, 3, nt), dtype=np.complex128)
         ur = np.zeros((3, 3, nt), dtype=np.complex128)
         tags = self.mesh.tet_tags
+        xs, ys, zs = self.mesh.centers
         for tag in np.unique(tags):
             mat = self.materials.get(int(tag), AIR)
-            ids = tags == tag
-            er[:, :, ids] = mat.ermat[:, :, None]
-            ur[:, :, ids] = mat.urmat[:, :, None]
+            ids = np.flatnonzero(tags == tag)
+            er[:, :, ids] = mat.fer3d_mat(xs[ids], ys[ids], zs[ids])
+            ur[:, :, ids] = mat.fur3d_mat(xs[ids], ys[ids], zs[ids])
         return er, ur
 
     def problem(self):
         """The device-resident frequency problem (built once per mesh/material/BC set)."""
         from .fdsolve import FrequencyProblem
-        key = (id(self.mesh), tuple(sorted((k, v.er, v.ur, v.tand) for k, v in self.materials.items())),
+        key = (id(self.mesh), tuple(sorted((k, v.key()) for k, v in self.materials.items())),
                tuple(id(b) for b in self.boundary_conditions), self.exact_face_mass, self.reference_geometry)
         if self._problem is None or self._problem_key != key:
             er, ur = self.material_tensors()
