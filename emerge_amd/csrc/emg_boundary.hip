@@ -122,6 +122,11 @@ __global__ void __launch_bounds__(128) k_interp(int64_t npts, const int32_t *__r
     const int64_t pt = gid % npts;
     const int sol = (int)(gid / npts);
     const int t = ptet[pt];
+    if (t < 0) {                                    // outside the mesh: zero field, as the reference
+        cplx *z = E + ((int64_t)sol * npts + pt) * 3;
+        z[0] = cplx(); z[1] = cplx(); z[2] = cplx();
+        return;
+    }
     const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
     const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
     double P[4][3];
@@ -169,6 +174,156 @@ __global__ void __launch_bounds__(128) k_interp(int64_t npts, const int32_t *__r
     }
     cplx *o = E + ((int64_t)sol * npts + pt) * 3;
     o[0] = ex; o[1] = ey; o[2] = ez;
+}
+
+// curl E at point `pt` of solution `sol` times c[tet] (c complex per tet, NULL = 1): for
+// N = s l_p (l_q grad l_r - l_r grad l_q),
+// curl N = s (l_q grad l_p x grad l_r - l_r grad l_p x grad l_q + 2 l_p grad l_q x grad l_r).
+__global__ void __launch_bounds__(128) k_interp_curl(int64_t npts, const int32_t *__restrict__ ptet,
+                                                     const double *__restrict__ bary, const double *__restrict__ nodes,
+                                                     const int32_t *__restrict__ tets, const int32_t *__restrict__ tet_ent,
+                                                     int64_t n_ent, int nsol, const cplx *__restrict__ x,
+                                                     const cplx *__restrict__ cscale, cplx *__restrict__ H) {
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (gid >= npts * nsol) return;
+    const int64_t pt = gid % npts;
+    const int sol = (int)(gid / npts);
+    const int t = ptet[pt];
+    cplx *o = H + ((int64_t)sol * npts + pt) * 3;
+    if (t < 0) {
+        o[0] = cplx(); o[1] = cplx(); o[2] = cplx();
+        return;
+    }
+    const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
+    const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+    double P[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) P[i][c] = nodes[3 * (int64_t)vid[i] + c];
+    double e1[3], e2[3], e3[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) { e1[c] = P[1][c] - P[0][c]; e2[c] = P[2][c] - P[0][c]; e3[c] = P[3][c] - P[0][c]; }
+    double g[4][3];
+    g[1][0] = e2[1] * e3[2] - e2[2] * e3[1]; g[1][1] = e2[2] * e3[0] - e2[0] * e3[2]; g[1][2] = e2[0] * e3[1] - e2[1] * e3[0];
+    g[2][0] = e3[1] * e1[2] - e3[2] * e1[1]; g[2][1] = e3[2] * e1[0] - e3[0] * e1[2]; g[2][2] = e3[0] * e1[1] - e3[1] * e1[0];
+    g[3][0] = e1[1] * e2[2] - e1[2] * e2[1]; g[3][1] = e1[2] * e2[0] - e1[0] * e2[2]; g[3][2] = e1[0] * e2[1] - e1[1] * e2[0];
+    const double det = e1[0] * g[1][0] + e1[1] * g[1][1] + e1[2] * g[1][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        g[0][c] = -(g[1][c] + g[2][c] + g[3][c]) / det;
+        g[1][c] /= det; g[2][c] /= det; g[3][c] /= det;
+    }
+    auto crs = [&](int u, int v, int c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        return g[u][c1] * g[v][c2] - g[u][c2] * g[v][c1];
+    };
+    const double lam[4] = {bary[4 * pt], bary[4 * pt + 1], bary[4 * pt + 2], bary[4 * pt + 3]};
+    const cplx *xs = x + (int64_t)sol * 2 * n_ent;
+    cplx hx, hy, hz;
+#pragma unroll
+    for (int s = 0; s < 10; s++) {
+        const int a = slot_vertex(s, 0), b = slot_vertex(s, 1), c = slot_vertex(s, 2);
+        const bool face = s >= 6;
+        const int ent = tet_ent[(int64_t)t * 10 + s];
+        auto len = [&](int u, int v) {
+            return sqrt((P[u][0] - P[v][0]) * (P[u][0] - P[v][0]) + (P[u][1] - P[v][1]) * (P[u][1] - P[v][1]) +
+                        (P[u][2] - P[v][2]) * (P[u][2] - P[v][2]));
+        };
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const int p = (m == 0) ? (face ? b : a) : (face ? c : b);
+            const int q = (m == 0 && face) ? c : b;
+            const int r = a;
+            const double sc = (m == 0 && face) ? -len(a, c) : len(a, b);
+            const cplx coef = xs[(int64_t)m * n_ent + ent];
+            double cv[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                cv[k] = sc * (lam[q] * crs(p, r, k) - lam[r] * crs(p, q, k) + 2.0 * lam[p] * crs(q, r, k));
+            hx += coef * cv[0];
+            hy += coef * cv[1];
+            hz += coef * cv[2];
+        }
+    }
+    const cplx cs = cscale ? cscale[t] : cplx(1.0, 0.0);
+    o[0] = hx * cs; o[1] = hy * cs; o[2] = hz * cs;
+}
+
+// Point location on a uniform grid of cells: each point tests the tets listed for its cell
+// (ascending tet id) and keeps the last one that holds it, with the reference's inside test
+// (fem/mth/tet.py:423-426: local coordinates >= -1e-6, their sum <= 1.00000001).  tet -1
+// and zero barycentric coordinates for points outside every candidate.
+__global__ void __launch_bounds__(256) k_locate(int64_t npts, const double *__restrict__ pts, const double *__restrict__ grid,
+                                                const int32_t *__restrict__ cell_ptr, const int32_t *__restrict__ cell_tets,
+                                                const double *__restrict__ nodes, const int32_t *__restrict__ tets,
+                                                int32_t *__restrict__ out_tet, double *__restrict__ out_bary) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= npts) return;
+    const double px = pts[3 * k], py = pts[3 * k + 1], pz = pts[3 * k + 2];
+    // grid: origin (3), inverse cell size (3), dims (3, as doubles)
+    const int nx = (int)grid[6], ny = (int)grid[7], nz = (int)grid[8];
+    const double fx = (px - grid[0]) * grid[3], fy = (py - grid[1]) * grid[4], fz = (pz - grid[2]) * grid[5];
+    int best = -1;
+    double bl[4] = {0.0, 0.0, 0.0, 0.0};
+    if (fx >= 0.0 && fy >= 0.0 && fz >= 0.0 && fx < nx && fy < ny && fz < nz) {
+        const int64_t cell = ((int64_t)fz * ny + (int)fy) * nx + (int)fx;
+        for (int32_t i = cell_ptr[cell]; i < cell_ptr[cell + 1]; i++) {
+            const int t = cell_tets[i];
+            const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
+            const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+            double P[4][3];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) P[v][c] = nodes[3 * (int64_t)vid[v] + c];
+            double e[3][3], r[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                e[0][c] = P[1][c] - P[0][c]; e[1][c] = P[2][c] - P[0][c]; e[2][c] = P[3][c] - P[0][c];
+            }
+            r[0] = px - P[0][0]; r[1] = py - P[0][1]; r[2] = pz - P[0][2];
+            // local coordinates u = T^{-1} r with T = [e0 e1 e2] as columns (Cramer's rule)
+            auto det3 = [](const double *a, const double *b, const double *c) {
+                return a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                       a[2] * (b[0] * c[1] - b[1] * c[0]);
+            };
+            const double D = det3(e[0], e[1], e[2]);
+            if (D == 0.0) continue;
+            const double u0 = det3(r, e[1], e[2]) / D, u1 = det3(e[0], r, e[2]) / D, u2 = det3(e[0], e[1], r) / D;
+            if (u0 >= -1e-6 && u1 >= -1e-6 && u2 >= -1e-6 && u0 + u1 + u2 <= 1.00000001) {
+                best = t;
+                bl[0] = 1.0 - u0 - u1 - u2; bl[1] = u0; bl[2] = u1; bl[3] = u2;
+            }
+        }
+    }
+    out_tet[k] = best;
+#pragma unroll
+    for (int j = 0; j < 4; j++) out_bary[4 * k + j] = bl[j];
+}
+
+extern "C" int emg_interp_curl_points(int64_t npts, const int32_t *d_ptet, const double *d_bary, const double *d_nodes,
+                                      const int32_t *d_tets, const int32_t *d_tet_ent, int64_t n_ent, int nsol,
+                                      const double *d_x, const double *d_cscale, double *d_H, void *stream) {
+    EMG_CHECK(d_ptet && d_bary && d_nodes && d_tets && d_tet_ent && d_x && d_H, "emg_interp_curl_points: null argument");
+    if (npts == 0 || nsol == 0) return 0;
+    k_interp_curl<<<emg::grid_for(npts * nsol, 128), 128, 0, (hipStream_t)stream>>>(
+        npts, d_ptet, d_bary, d_nodes, d_tets, d_tet_ent, n_ent, nsol, reinterpret_cast<const cplx *>(d_x),
+        reinterpret_cast<const cplx *>(d_cscale), reinterpret_cast<cplx *>(d_H));
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+extern "C" int emg_locate_points(int64_t npts, const double *d_pts, const double *d_grid, const int32_t *d_cell_ptr,
+                                 const int32_t *d_cell_tets, const double *d_nodes, const int32_t *d_tets,
+                                 int32_t *d_tet, double *d_bary, void *stream) {
+    EMG_CHECK(d_pts && d_grid && d_cell_ptr && d_cell_tets && d_nodes && d_tets && d_tet && d_bary,
+              "emg_locate_points: null argument");
+    if (npts == 0) return 0;
+    k_locate<<<emg::grid_for(npts, 256), 256, 0, (hipStream_t)stream>>>(npts, d_pts, d_grid, d_cell_ptr, d_cell_tets,
+                                                                       d_nodes, d_tets, d_tet, d_bary);
+    EMG_LAUNCH_CHECK();
+    return 0;
 }
 
 extern "C" int emg_interp_points(int64_t npts, const int32_t *d_ptet, const double *d_bary, const double *d_nodes,

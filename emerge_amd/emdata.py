@@ -74,6 +74,8 @@ class EMDataSet:
         self.mode = None
         self.beta = None
         self._mode_field = None
+        self._basis = None                  # emerge_amd.fields.PointLocator of the mesh
+        self.Ex = self.Ey = self.Ez = self.Hx = self.Hy = self.Hz = None
         for k, v in kw.items():
             setattr(self, k, v)
 
@@ -93,6 +95,31 @@ class EMDataSet:
         self.excitation = {k: 0.0 for k in self._fields}
         if self.port_modes:
             self.excitation[self.port_modes[0].port_number] = 1.0 + 0j
+
+    def interpolate(self, xs, ys, zs, freq=None) -> "EMDataSet":
+        """E and H of this dataset's field at the points (fem/physics/edm/emdata.py:181-199):
+        sets Ex, Ey, Ez and Hx, Hy, Hz (shaped like xs) with H = curl E / (-j 2 pi f mu0 mu_r)
+        per tet, and returns self.  The field is the excitation-weighted sum of the stored port
+        solutions (needs store_fields) or the mode field of an eigen dataset."""
+        if self._basis is None:
+            raise ValueError("this dataset has no mesh attached to interpolate on")
+        xs, ys, zs = (np.asarray(v, dtype=np.float64) for v in (xs, ys, zs))
+        shp = xs.shape
+        pts = np.stack([xs.ravel(), ys.ravel(), zs.ravel()])
+        ur = np.broadcast_to(np.asarray(self.ur if self.ur is not None else 1.0), (self._basis.mesh.n_tets,))
+        const = 1.0 / (-1j * 2 * np.pi * self.freq * (ur * 4 * np.pi * 1e-7))
+        E, H = self._basis.fields(np.asarray(self._field, dtype=np.complex128), pts, curl_scale=const)
+        self.Ex, self.Ey, self.Ez = (E[i].reshape(shp) for i in range(3))
+        self.Hx, self.Hy, self.Hz = (H[i].reshape(shp) for i in range(3))
+        return self
+
+    @property
+    def E(self):
+        return self.Ex, self.Ey, self.Ez
+
+    @property
+    def H(self):
+        return self.Hx, self.Hy, self.Hz
 
     @property
     def _field(self):
@@ -130,6 +157,7 @@ class EMSimData:
 
     def new(self, **vars) -> EMDataSet:
         ds = EMDataSet(**vars)
+        ds._basis = self._basis
         self.datasets.append(ds)
         return ds
 

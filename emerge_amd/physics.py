@@ -129,6 +129,13 @@ class Electrodynamics3D:
             ur[:, :, ids] = mat.fur3d_mat(xs[ids], ys[ids], zs[ids])
         return er, ur
 
+    def locator(self):
+        """Point locator of the mesh for EMDataSet.interpolate (built on first use)."""
+        from .fields import LazyLocator
+        if getattr(self, "_locator", None) is None or self._locator.mesh is not self.mesh:
+            self._locator = LazyLocator(self.mesh)
+        return self._locator
+
     def problem(self):
         """The device-resident frequency problem (built once per mesh/material/BC set)."""
         from .fdsolve import FrequencyProblem
@@ -166,7 +173,7 @@ class Electrodynamics3D:
             solver.learned = st.learned
         ports = prob.ports
         nums = [p.port_number for p in ports]
-        data = EMSimData()
+        data = EMSimData(basis=self.locator())
         er, ur = prob.er, prob.ur
         prev: dict[int, object] = {}
         t0 = time.perf_counter()
@@ -260,7 +267,7 @@ class Electrodynamics3D:
                                  preconditioner=st.preconditioner, verbose=getattr(st, "verbose", False),
                                  poly_degree=st.poly_degree)
         lam, X, resid = arn.run(num_sols, spurious_rel=spurious_rel)
-        data = EMSimData()
+        data = EMSimData(basis=self.locator())
         for i in range(lam.shape[0]):
             f = float(eigen_frequencies(lam[i].real))
             ds = data.new(freq=f, k0=float(np.sqrt(lam[i].real)), mode=i + 1)

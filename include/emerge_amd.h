@@ -199,6 +199,24 @@ int emg_interp_points(int64_t npts, const int32_t *d_ptet, const double *d_bary,
                       const int32_t *d_tets, const int32_t *d_tet_ent, int64_t n_ent, int nsol,
                       const double *d_x, double *d_E, void *stream);
 
+/* curl of the Nedelec-2 field of nsol solutions at tet-tagged points (arguments as
+ * emg_interp_points), times d_cscale[tet] (nt complex128, NULL = 1): d_H nsol x npts x 3
+ * complex128.  Replaces ned2_tet_interp_curl (fem/mth/tet.py:499-626), used by
+ * EMDataSet.interpolate for H (fem/physics/edm/emdata.py:181-199).  Points with tet
+ * -1 (outside the mesh) get zero, as in the reference (and in emg_interp_points).  */
+int emg_interp_curl_points(int64_t npts, const int32_t *d_ptet, const double *d_bary, const double *d_nodes,
+                           const int32_t *d_tets, const int32_t *d_tet_ent, int64_t n_ent, int nsol,
+                           const double *d_x, const double *d_cscale, double *d_H, void *stream);
+/* Tet holding each of npts points (d_pts npts x 3) and its barycentric coordinates
+ * (d_bary npts x 4, canonical vertex order), by a uniform grid of cells: d_grid = origin
+ * (3), inverse cell size (3), cell counts (3, as doubles); d_cell_ptr (ncells+1) and
+ * d_cell_tets list the tets overlapping each cell in ascending id.  The last tet that
+ * holds a point wins, with the reference's tolerances (fem/mth/tet.py:423-426); -1 if
+ * none.  Replaces the brute-force search over all tets in ned2_tet_interp.          */
+int emg_locate_points(int64_t npts, const double *d_pts, const double *d_grid, const int32_t *d_cell_ptr,
+                      const int32_t *d_cell_tets, const double *d_nodes, const int32_t *d_tets, int32_t *d_tet,
+                      double *d_bary, void *stream);
+
 /* ------------------------------------------------------------------ batched sweep
  * A block of nvec (frequency, port) pairs over the shared pattern; replaces the
  * per-(frequency, port) assemble + solve loop of frequency_domain

@@ -213,6 +213,47 @@ class OracleProblem:
             out[:, inside] = E
         return out
 
+    # fem/mth/tet.py:499-626 (ned2_tet_interp_curl), the analytic curl of the basis of
+    # tet.py:451-492: for N = s l_p (l_q grad l_r - l_r grad l_q),
+    # curl N = s (l_q grad l_p x grad l_r - l_r grad l_p x grad l_q + 2 l_p grad l_q x grad l_r)
+    def interpolate_curl(self, x, pts, tet_ids=None, c=None):
+        om, f = self.om, self.field
+        nodes = om.nodes
+        ne = om.n_edges
+        out = np.zeros((3, pts.shape[1]), dtype=np.complex128)
+        if tet_ids is None:
+            tet_ids = np.arange(om.n_tets)
+        for itet in tet_ids:
+            vid = om.tets[:, itet]
+            v = nodes[:, vid]
+            T = np.stack([v[:, 1] - v[:, 0], v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]], axis=1)
+            loc = np.linalg.pinv(T) @ (pts - v[:, :1])
+            inside = (loc.sum(0) <= 1.00000001) & np.all(loc >= -1e-6, axis=0)
+            if not inside.any():
+                continue
+            lam = np.vstack([1 - loc[:, inside].sum(0), loc[:, inside]])
+            G = np.linalg.inv(T)
+            G = np.vstack([-G.sum(0), G])
+            Et = x[f.tet_to_field[:, itet]]
+            lem = O.local_mapping(vid, om.edges[:, f.tet_to_field[:6, itet]])
+            ltm = O.local_mapping(vid, om.tris[:, f.tet_to_field[6:10, itet] - ne])
+            cr = lambda u, w: np.cross(G[u], G[w])[:, None]
+
+            def curl(sc, p, q, r):
+                return sc * (lam[q] * cr(p, r) - lam[r] * cr(p, q) + 2 * lam[p] * cr(q, r))
+            H = np.zeros((3, int(inside.sum())), dtype=np.complex128)
+            for ie in range(6):
+                a, b_ = lem[:, ie]
+                L = np.linalg.norm(v[:, a] - v[:, b_])
+                H += Et[ie] * curl(L, a, b_, a) + Et[10 + ie] * curl(L, b_, b_, a)
+            for ie in range(4):
+                a, b_, cc = ltm[:, ie]
+                L1 = np.linalg.norm(v[:, a] - v[:, cc])
+                L2 = np.linalg.norm(v[:, a] - v[:, b_])
+                H += Et[6 + ie] * curl(-L1, b_, cc, a) + Et[16 + ie] * curl(L2, cc, b_, a)
+            out[:, inside] = H * (1.0 if c is None else c[itet])
+        return out
+
     # fem/physics/edm/emfreq3d.py:734-779
     def s_data(self, bc, x, k0):
         if bc.kind == "lumped":

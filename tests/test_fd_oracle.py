@@ -181,3 +181,27 @@ def test_oracle_waveguide_mode_power():
     prob = fd.OracleProblem(mesh, er, ur, [p1])
     _, mp = prob.s_data(prob.bcs[0], np.zeros(prob.N, dtype=np.complex128), 2 * np.pi * 10e9 / C0)
     assert abs(mp - 377 / 376.730313412) < 2e-5
+
+
+def test_oracle_curl_interpolation_by_finite_differences():
+    """The oracle's analytic curl (tet.py:499-626) equals central differences of its own
+    interpolated E (tet.py:371-497) inside a tet, for a random field."""
+    mesh = meshgen.structured_box(2, 2, 2, size=(1.0, 0.8, 0.6), jitter=0.1, seed=3)
+    er, ur = O.material_tensors(np.ones(mesh.n_tets, complex))
+    prob = fd.OracleProblem(mesh, er, ur, [], order_seed=5)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(prob.N) + 1j * rng.standard_normal(prob.N)
+    t = 7
+    v = prob.om.nodes[:, prob.om.tets[:, t]]
+    p0 = v @ np.array([0.3, 0.25, 0.2, 0.25])
+    h = 1e-5
+    J = np.zeros((3, 3), dtype=complex)                       # J[i, j] = dE_i / dx_j
+    for j in range(3):
+        d = np.zeros(3)
+        d[j] = h
+        pts = np.stack([p0 + d, p0 - d], axis=1)
+        E = prob.interpolate(x, pts, [t])
+        J[:, j] = (E[:, 0] - E[:, 1]) / (2 * h)
+    fdcurl = np.array([J[2, 1] - J[1, 2], J[0, 2] - J[2, 0], J[1, 0] - J[0, 1]])
+    an = prob.interpolate_curl(x, p0[:, None], [t])[:, 0]
+    assert np.abs(an - fdcurl).max() <= 1e-6 * np.abs(an).max()
