@@ -43,6 +43,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_robin_blocks": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P, P]),
     "emg_interp_points": (ctypes.c_int, [I64, P, P, P, P, P, I64, ctypes.c_int, P, P, P]),
     "emg_interp_curl_points": (ctypes.c_int, [I64, P, P, P, P, P, I64, ctypes.c_int, P, P, P, P]),
+    "emg_farfield": (ctypes.c_int, [I64, P, P, P, P, I64, P, P, F64, P, P, P]),
     "emg_locate_points": (ctypes.c_int, [I64, P, P, P, P, P, P, P, P, P]),
     "emg_solver_set_weights": (ctypes.c_int, [P, P, P]),
     "emg_solver_smoother_parts": (ctypes.c_int, [P, P, P, P, P]),

@@ -177,6 +177,12 @@ class Mesh3D:
             vol_tags = [int(vol_tags)]
         return np.flatnonzero(np.isin(self.tet_tags, np.asarray(vol_tags)))
 
+    def boundary_surface(self, face_tags, origin):
+        """The tagged triangles as a SurfaceMesh with normals pointing away from origin
+        (fem/mesh3d.py:450-456)."""
+        from .farfield import SurfaceMesh
+        return SurfaceMesh(self, self.get_triangles(face_tags), origin)
+
     def boundary_triangles(self) -> np.ndarray:
         """Triangles with a single adjacent tet (the exterior surface)."""
         return np.flatnonzero(self.tri_to_tet[1] < 0)

@@ -217,6 +217,16 @@ int emg_locate_points(int64_t npts, const double *d_pts, const double *d_grid, c
                       const int32_t *d_cell_tets, const double *d_nodes, const int32_t *d_tets, int32_t *d_tet,
                       double *d_bary, void *stream);
 
+/* Stratton-Chu far field (fem/physics/edm/sc.py:27-180): ns surface samples at d_pts
+ * (ns x 3) with area-weighted normals d_wn (ns x 3) carrying E and H (ns x 3 complex128);
+ * for ndir directions r = (cos t cos p, cos t sin p, sin t) (d_theta, d_phi, radians):
+ *   E(r) = -j k0/(4 pi) sum_j r x [(w_j x E_j - Z0 r x (w_j x H_j)) exp(j k0 r.v_j)],
+ *   H(r) = r x E(r) / Z0; d_Eout, d_Hout ndir x 3 complex128.  FP64 (the reference
+ * evaluates the same sum in complex64).                                          */
+int emg_farfield(int64_t ns, const double *d_pts, const double *d_wn, const double *d_E, const double *d_H,
+                 int64_t ndir, const double *d_theta, const double *d_phi, double k0, double *d_Eout,
+                 double *d_Hout, void *stream);
+
 /* ------------------------------------------------------------------ batched sweep
  * A block of nvec (frequency, port) pairs over the shared pattern; replaces the
  * per-(frequency, port) assemble + solve loop of frequency_domain
