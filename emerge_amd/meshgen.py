@@ -186,3 +186,77 @@ def cavity_block(nx=48, ny=32, nz=24, seed=3, size=(30e-3, 20e-3, 15e-3), block=
         return np.where(inside, BLOCK_TAG, 1)
 
     return structured_box(nx, ny, nz, size=size, jitter=0.05, seed=seed, tag_fn=tag)
+
+
+PATCH_TAG, GROUND_TAG, FEED_PORT_TAG = 10, 11, 12
+SUBSTRATE_TAG = 2
+
+
+def tensor_nodes(xs, ys, zs):
+    """(3, nv) vertices of a tensor grid of coordinate arrays, x fastest, and the id grid."""
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), Z.ravel()])
+    ids = np.arange(nodes.shape[1]).reshape(len(zs), len(ys), len(xs)).transpose(2, 1, 0)
+    return nodes, ids
+
+
+def patch_antenna(cell=1.25e-3, half=45e-3, below=12.5e-3, above=25e-3, h_sub=1.6e-3, sub_layers=2, ground=60e-3,
+                  patch_w=38e-3, patch_l=29e-3, line_w=2.5e-3, inset=8e-3, gap=1.0e-3, port_y=-27.5e-3,
+                  jitter=0.05, seed=4, er_sub=4.4, scale_w=1.0, scale_l=1.0):
+    """Microstrip patch antenna in an air box: a tensor grid of Kuhn tets over
+    [-half, half]^2 x [-h_sub - below, above] with a uniform `cell` in x, y and above
+    the substrate, `sub_layers` layers in the h_sub substrate (|x|, |y| <= ground / 2,
+    -h_sub < z < 0, volume tag SUBSTRATE_TAG).  Interior triangles carry the conductors:
+    GROUND_TAG (z = -h_sub, |x|, |y| <= ground / 2), PATCH_TAG (z = 0: the patch_w x
+    patch_l patch, centred, with an inset feed notch of depth `inset` and gap `gap` beside a
+    line_w wide line that runs to y = port_y) and FEED_PORT_TAG (the line_w x h_sub strip
+    at y = port_y between ground and line).  The outer faces keep tags 1..6 (for an
+    absorbing boundary).  scale_w / scale_l perturb the patch size (training variants).
+    Vertices not on the conductor planes are jittered by `jitter` of the local spacing."""
+    rng = np.random.default_rng(seed)
+    n_xy = int(round(2 * half / cell))
+    xs = np.linspace(-half, half, n_xy + 1)
+    ys = xs.copy()
+    n_below = int(round(below / cell))
+    n_above = int(round(above / cell))
+    zs = np.concatenate([np.linspace(-h_sub - below, -h_sub, n_below + 1)[:-1],
+                         np.linspace(-h_sub, 0.0, sub_layers + 1)[:-1], np.linspace(0.0, above, n_above + 1)])
+    nodes, ids = tensor_nodes(xs, ys, zs)
+    if jitter > 0:
+        inner = ids[1:-1, 1:-1, 1:-1].ravel()
+        z = nodes[2, inner]
+        y = nodes[1, inner]
+        free = (np.abs(z + h_sub) > 1e-12) & (np.abs(z) > 1e-12) & (np.abs(y - port_y) > 1e-12)
+        inner = inner[free]
+        dz = np.diff(zs)
+        kz = np.clip(np.searchsorted(zs, nodes[2, inner]) - 1, 0, len(dz) - 1)
+        hz = np.minimum(dz[kz], dz[np.minimum(kz + 1, len(dz) - 1)])
+        d = rng.uniform(-jitter, jitter, size=(3, inner.shape[0]))
+        nodes = nodes.copy()
+        nodes[0, inner] += d[0] * cell
+        nodes[1, inner] += d[1] * cell
+        nodes[2, inner] += d[2] * hz
+    tets = orient_positive(nodes, kuhn_tets(ids))
+    cent = nodes[:, tets].mean(axis=1)
+    g2 = ground / 2
+    sub = (np.abs(cent[0]) < g2) & (np.abs(cent[1]) < g2) & (cent[2] > -h_sub) & (cent[2] < 0)
+    tags = np.where(sub, SUBSTRATE_TAG, 1)
+    size = (2 * half, 2 * half, zs[-1] - zs[0])
+    origin = (-half, -half, zs[0])
+    mesh = Mesh3D(nodes, tets, tet_tags=tags, face_tags=box_face_tags(nodes, tets, size, origin))
+    # interior triangles of the conductor planes (grid planes: their vertices are not jittered)
+    s = mesh.tris
+    fz = nodes[2, s]
+    fc = mesh.tri_centers
+    pw, pl = patch_w * scale_w / 2, patch_l * scale_l / 2
+    lw = line_w / 2
+    on_z0 = np.all(np.abs(fz) < 1e-12, axis=0)
+    in_patch = (np.abs(fc[0]) < pw) & (np.abs(fc[1]) < pl)
+    notch = (np.abs(fc[0]) < lw + gap) & (fc[1] < -pl + inset) & (fc[1] > -pl)
+    line = (np.abs(fc[0]) < lw) & (fc[1] > port_y) & (fc[1] < -pl + inset)
+    mesh.face_tags[PATCH_TAG] = np.flatnonzero(on_z0 & ((in_patch & ~notch) | line))
+    on_g = np.all(np.abs(fz + h_sub) < 1e-12, axis=0)
+    mesh.face_tags[GROUND_TAG] = np.flatnonzero(on_g & (np.abs(fc[0]) < g2) & (np.abs(fc[1]) < g2))
+    on_p = np.all(np.abs(nodes[1, s] - port_y) < 1e-12, axis=0)
+    mesh.face_tags[FEED_PORT_TAG] = np.flatnonzero(on_p & (np.abs(fc[0]) < lw) & (fc[2] > -h_sub) & (fc[2] < 0))
+    return mesh

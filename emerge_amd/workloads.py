@@ -14,6 +14,10 @@ standing in for gmsh), plus the workload's frequencies or eigen target.
 * ``cavity``: PEC box 30 x 20 x 15 mm with a lossless eps_r = 6 block
               10 x 10 x 5 mm 3 mm from a corner, 48 x 32 x 24 hexes
               (221,184 tets, seed 3); modes nearest 5 GHz.
+* ``patch``:  microstrip patch 38 x 29 mm (inset feed, 2.5 mm line) on a 60 x 60 mm
+              ground with an eps_r 4.4, tan d 0.02, 1.6 mm substrate, in a 90 x 90 x 39 mm
+              air box with a first-order absorbing boundary, a 50-ohm lumped port across
+              the substrate at the line's end; 995,328 tets (seed 4); 101 points over 2-3 GHz.
 """
 from __future__ import annotations
 
@@ -66,4 +70,24 @@ def cavity(nx=48, ny=32, nz=24, seed=3):
     phys = Electrodynamics3D(mesh)
     phys.set_material(meshgen.BLOCK_TAG, Material(er=6.0))
     phys._bc_initialized = True
+    return phys
+
+
+PATCH_FREQS = tuple(np.linspace(2e9, 3e9, 101))
+
+
+def patch(seed=4, freqs=PATCH_FREQS, er_sub=4.4, tand=0.02, scale_w=1.0, scale_l=1.0, **mesh_kw):
+    from .bc import AbsorbingBoundary
+    kw = dict(mesh_kw)
+    mesh = meshgen.patch_antenna(seed=seed, scale_w=scale_w, scale_l=scale_l, **kw)
+    h_sub = kw.get("h_sub", 1.6e-3)
+    port_y = kw.get("port_y", -27.5e-3)
+    line_w = kw.get("line_w", 2.5e-3)
+    phys = Electrodynamics3D(mesh)
+    phys.set_material(meshgen.SUBSTRATE_TAG, Material(er=er_sub, tand=tand))
+    cs = CoordinateSystem((1, 0, 0), (0, 0, 1), (0, -1, 0), (0.0, port_y, -h_sub / 2))
+    port = LumpedPort([meshgen.FEED_PORT_TAG], 1, width=line_w, height=h_sub, direction=(0, 0, 1), cs=cs, Z0=50)
+    abc = AbsorbingBoundary([meshgen.XMIN, meshgen.XMAX, meshgen.YMIN, meshgen.YMAX, meshgen.ZMIN, meshgen.ZMAX])
+    phys.assign(port, abc)
+    phys.set_frequency(list(freqs))
     return phys
