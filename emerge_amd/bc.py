@@ -367,3 +367,83 @@ class LumpedPort(PortBC):
 
     def mode_shape(self, x_local, y_local):
         return self.port_mode_3d(x_local, y_local, 0.0)
+
+
+class PortMode:
+    """One eigenmode of a modal port (fem/bc.py:302-327): the 14-DOF face field, its
+    propagation constant, the normalisation to unit power and the mode type."""
+
+    def __init__(self, field, beta, k0, freq, power, modetype: str, TEM: bool, eigenvalue=None):
+        self.modefield = field
+        self.beta = beta
+        self.k0 = k0
+        self.freq = freq
+        self.TEM = TEM
+        self.modetype = modetype
+        self.eigenvalue = eigenvalue
+        self.neff = beta / k0
+        self.energy = float(np.mean(np.abs(field) ** 2))
+        self.norm_factor = float(np.sqrt(1 / np.abs(power)))                     # set_power, fem/bc.py:325-327
+        self.power = power
+
+
+class ModalPort(PortBC):
+    """Port whose mode comes from an eigenmode analysis of its face (fem/bc.py:329-500):
+    run Electrodynamics3D.modal_analysis(port) before the sweep.  `cs` is the face frame
+    (zax = the face normal); the reference derives it from the face when omitted."""
+    _include_stiff = True
+    _include_mass = False
+    _include_force = True
+
+    def __init__(self, tags, port_number: int, active: bool = False, cs: CoordinateSystem | None = None,
+                 power: float = 1.0):
+        super().__init__(tags)
+        if cs is None:
+            raise ValueError("ModalPort needs the face coordinate system here (zax = face normal)")
+        self.port_number = port_number
+        self.active = active
+        self.power = power
+        self.cs = cs
+        self.selected_mode = 0
+        self.modes: list[PortMode] = []
+        self.face = None                   # emerge_amd.modal.PortFace after modal_analysis
+
+    @property
+    def nmodes(self) -> int:
+        return len(self.modes)
+
+    def get_mode(self, i=None) -> PortMode:
+        return self.modes[self.selected_mode if i is None else i]
+
+    @property
+    def modetype(self):                                                      # fem/bc.py:376-379
+        return self.get_mode().modetype
+
+    def get_beta(self, k0):                                                  # fem/bc.py:444-453
+        m = self.get_mode()
+        if m.TEM:
+            return m.beta / m.k0 * k0
+        freq = k0 * C0 / (2 * np.pi)
+        return np.sqrt(m.beta ** 2 + k0 ** 2 * (1 - (m.freq / freq) ** 2))
+
+    def get_gamma(self, k0):
+        return 1j * self.get_beta(k0)
+
+    def port_mode_3d(self, x_local, y_local, k0=None):
+        """Mode E in local components at local face points (fem/bc.py:469-484)."""
+        xg, yg, zg = self.cs.in_global_cs(np.asarray(x_local), np.asarray(y_local), 0 * np.asarray(x_local))
+        E = self.port_mode_3d_global(xg, yg, zg, k0)
+        return np.array(self.cs.in_local_basis(E[0], E[1], E[2]))
+
+    def port_mode_3d_global(self, xg, yg, zg, k0=None):                      # fem/bc.py:486-500
+        m = self.get_mode()
+        return np.sqrt(self.power) * m.norm_factor * self.face.field_at(m.modefield, xg, yg, zg)
+
+    def get_Uinc(self, x_local, y_local, k0):
+        return -2j * self.get_beta(k0) * self.port_mode_3d(x_local, y_local, k0)
+
+    def force_scale(self, k0):
+        return -2j * self.get_beta(k0)
+
+    def mode_shape(self, x_local, y_local):
+        return self.port_mode_3d(x_local, y_local)

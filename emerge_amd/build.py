@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemerge_amd.so")
-SOURCES = ["emg_assembly.hip", "emg_krylov.hip", "emg_boundary.hip", "emg_sweep.hip", "emg_gnn.hip", "emg_perm.hip", "emg_farfield.hip"]
+SOURCES = ["emg_assembly.hip", "emg_krylov.hip", "emg_boundary.hip", "emg_sweep.hip", "emg_gnn.hip", "emg_perm.hip", "emg_farfield.hip", "emg_port.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -21,11 +21,14 @@ HIPCC_FLAGS = [
     "-Wno-unused-result",
 ]
 
+# rocSOLVER (dense LU of the port-face modal pencil) and rocBLAS (its matrix products)
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+
 
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
-    cmd = ["hipcc", *HIPCC_FLAGS, "-I", os.path.join(os.path.dirname(HERE), "include"), "-o", tmp, *srcs]
+    cmd = ["hipcc", *HIPCC_FLAGS, "-I", os.path.join(os.path.dirname(HERE), "include"), "-o", tmp, *srcs, *LIBS]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

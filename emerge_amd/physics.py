@@ -238,6 +238,17 @@ class Electrodynamics3D:
         self.data = data
         return data
 
+    def modal_analysis(self, port, nmodes: int = 6, direct: bool = True, TEM: bool = False, target_kz=None,
+                       freq: float | None = None, face_block: bool = False):
+        """The mode(s) of a ModalPort from its face, on the GPU (emfreq3d.py:201-364); fills
+        port.modes and returns them.  `direct` is accepted for the reference's signature; the
+        GPU search always returns every mode in the reference's window."""
+        from .modal import modal_analysis
+        modes = modal_analysis(self, port, nmodes=nmodes, TEM=TEM, freq=freq, target_kz=target_kz,
+                               face_block=face_block)
+        self._problem = None
+        return modes
+
     def eigenmode(self, mesh: Mesh3D | None = None, solver=None, num_sols: int = 6,
                   target_frequency: float | None = None, ncv: int | None = None, inner_tol: float = 1e-12,
                   spurious_rel: float = 1e-3) -> EMSimData:

@@ -227,6 +227,62 @@ int emg_farfield(int64_t ns, const double *d_pts, const double *d_wn, const doub
                  int64_t ndir, const double *d_theta, const double *d_phi, double k0, double *d_Eout,
                  double *d_Hout, void *stream);
 
+/* ---- port-face modal analysis (fem/physics/edm/emfreq3d.py:201-364) ---- */
+#define EMG_PORT_FACE_BLOCK 1 /* fill the face-face entries of Dtt, which the reference's
+                                 generalized_matrix_GQ leaves zero (nedeleclegrange2.py:350-358) */
+
+/* Port element matrices and their dense assembly.  Replaces generalized_matrix_GQ and
+ * _matrix_builder (fem/physics/edm/nedeleclegrange2.py:224-418) with the NedelecLegrange2
+ * DOF order of fem/elements/nedleg2.py:54-61.  ntri triangles, d_verts (ntri x 2 x 3: x0 x1
+ * x2 y0 y1 y2 in the face frame, ascending vertex numbers), d_Ms = ur^-1 and d_Mm = er per
+ * triangle (ntri x 3 x 3 complex128, row-major), k0 in rad/m.  With d_A/d_B (n x n complex128,
+ * column-major, zeroed by the caller) every entry is added at (d_t2r[t,i], d_t2r[t,j]) of
+ * d_t2r (ntri x 14 int64: reduced DOF or -1 for a removed DOF); with d_Ael/d_Bel the
+ * element matrices are also written (ntri x 14 x 14, row-major).                       */
+int emg_port_assemble(int64_t ntri, const double *d_verts, const double *d_Ms, const double *d_Mm, double k0,
+                      int flags, const int64_t *d_t2r, int64_t n, double *d_A, double *d_B, double *d_Ael,
+                      double *d_Bel, void *stream);
+
+/* The face field at np points d_pts (np x 2, face frame): ned2_tri_interp_full and
+ * ned2_tri_interp_curl (fem/mth/tri.py:228-401), the last triangle holding a point wins,
+ * points outside every triangle get zeros.  d_tris (ntri x 3, entries < nnodes), d_t2f
+ * (ntri x 14, entries < nfield) int64, d_nodes (nnodes x 2) in the face frame, d_field
+ * (nfield) complex128; a point whose triangle has an index outside those sizes gets NaN.  d_E (np x 3 complex128) =
+ * d_basis (3 x 3 row-major, local -> global) applied to (Ex, Ey, Ez); with d_H, d_H =
+ * hconst * basis (diag(d_diadic[t]) * (dEz/dy + j beta Ey, dEz/dx - j beta Ex, curl_z Et)),
+ * d_diadic ntri x 3 x 3 complex128 indexed by the triangle number as the reference does
+ * (NedelecLegrange2.interpolate_Hf, fem/elements/nedleg2.py:92-105).                   */
+int emg_port_field(int64_t np, const double *d_pts, int64_t ntri, const int64_t *d_tris, int64_t nnodes,
+                   const double *d_nodes, const int64_t *d_t2f, int64_t nfield, const double *d_field, double beta,
+                   const double *d_diadic,
+                   const double *d_basis, double hconst_re, double hconst_im, double *d_E, double *d_H,
+                   void *stream);
+
+/* d_M = d_A - sigma d_B (n x n complex128, column-major; sigma = sigma_re + j sigma_im) and
+ * its rocSOLVER LU in place (d_ipiv: n int32).  Replaces the dense LAPACK factorisation
+ * inside scipy's eig (SolverLAPACK.eig, fem/solver.py:311-341); fails if the shifted
+ * matrix is exactly singular.                                                         */
+int emg_port_factor(int64_t n, const double *d_A, const double *d_B, double sigma_re, double sigma_im, double *d_M,
+                    int *d_ipiv, void *stream);
+
+/* d_Y = (A - sigma B)^{-1} B d_X for k columns (n x k complex128, column-major) from the LU
+ * of emg_port_factor: one step of inverse iteration on k vectors.                     */
+int emg_port_inverse_step(int64_t n, const double *d_M, const int *d_ipiv, const double *d_B, int64_t k,
+                          const double *d_X, double *d_Y, void *stream);
+
+/* m steps of Arnoldi on (A - sigma B)^{-1} B from the LU of emg_port_factor: d_V (n x (m+1)
+ * complex128, column 0 the start vector, normalised here) receives the orthonormal basis,
+ * d_H ((m+1) x m complex128, column-major) the Hessenberg matrix; a breakdown leaves zero
+ * columns and a zero subdiagonal entry.  Classical Gram-Schmidt, two passes.             */
+int emg_port_krylov(int64_t n, const double *d_M, const int *d_ipiv, const double *d_B, int m, double *d_V,
+                    double *d_H, void *stream);
+
+/* C = alpha op(A) op(B) + beta C, complex128 column-major, op 0 = N, 1 = T, 2 = C
+ * (rocBLAS zgemm): the Rayleigh-Ritz projections and Ritz vectors of the port solve.   */
+int emg_zgemm(int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha_re, double alpha_im,
+              const double *d_A, int64_t lda, const double *d_B, int64_t ldb, double beta_re, double beta_im,
+              double *d_C, int64_t ldc, void *stream);
+
 /* ------------------------------------------------------------------ batched sweep
  * A block of nvec (frequency, port) pairs over the shared pattern; replaces the
  * per-(frequency, port) assemble + solve loop of frequency_domain

@@ -270,7 +270,12 @@ class OracleProblem:
         tets = self.om.tri_to_tet[0, tri]
         er = self.er[:, :, tets]
         ur = self.ur[:, :, tets]
-        const = 1 / np.sqrt((ur[0, 0] + ur[1, 1] + ur[2, 2]) / (er[0, 0] + er[1, 1] + er[2, 2]))   # TEM
+        if bc.modetype == "TE":                                         # emfreq3d.py:770-775
+            const = 1 / ((ur[0, 0] + ur[1, 1] + ur[2, 2]) / 3)
+        elif bc.modetype == "TM":
+            const = 1 / ((er[0, 0] + er[1, 1] + er[2, 2]) / 3)
+        else:
+            const = 1 / np.sqrt((ur[0, 0] + ur[1, 1] + ur[2, 2]) / (er[0, 0] + er[1, 1] + er[2, 2]))
         dpts = gaus_quad_tri4()
         nodes = self.om.nodes
         P = np.einsum("cvt,vq->cqt", nodes[:, tv], dpts[1:])          # (3, nq, ntri)

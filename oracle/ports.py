@@ -58,13 +58,20 @@ class OraclePort:
         self.active = active
         self.line = None                      # (midpoints (3, n), segment vectors (3, n)) of a lumped port
 
+    modes = None            # kind "modal": list of mode dicts from oracle.modal.modal_analysis
+    port_field = None       # kind "modal": the port face's oracle.modal.OraclePortMode
+    selected_mode = 0
+
     @property
     def is_port(self):
-        return self.kind in ("waveguide", "lumped")
+        return self.kind in ("waveguide", "lumped", "modal")
 
     @property
     def robin(self):
-        return self.kind in ("abc", "waveguide", "lumped")
+        return self.kind in ("abc", "waveguide", "lumped", "modal")
+
+    def _mode(self):
+        return self.modes[self.selected_mode]
 
     @property
     def has_source(self):                                                  # _include_force
@@ -72,6 +79,12 @@ class OraclePort:
 
     # ---------------------------------------------------------------- constants
     def beta(self, k0):
+        if self.kind == "modal":                                           # fem/bc.py:444-453
+            m = self._mode()
+            if m["TEM"]:
+                return m["beta"] / m["k0"] * k0
+            f = k0 * C0 / (2 * np.pi)
+            return np.sqrt(m["beta"] ** 2 + k0 ** 2 * (1 - (m["freq"] / f) ** 2))
         if self.kind == "waveguide":                                       # fem/bc.py:571-575, mode (1, 0)
             a = self.dims[0]
             return np.sqrt(k0 * k0 - (np.pi / a) ** 2 + 0j)
@@ -86,8 +99,17 @@ class OraclePort:
     def voltage(self):                                                     # fem/bc.py:684-690
         return np.sqrt(2.0 * self.power * self.Z0)
 
+    @property
+    def modetype(self):
+        """fem/bc.py:203-205 ('TEM' for every fixed-mode port), 376-379 (the modal port's mode)."""
+        return self._mode()["modetype"] if self.kind == "modal" else "TEM"
+
     def zmode(self, k0):
-        """fem/bc.py:207-215 with the TEM branch every port takes (modetype is 'TEM')."""
+        """fem/bc.py:207-215."""
+        if self.modetype == "TE":
+            return k0 * C0 / self.beta(k0) * 4 * np.pi * 1e-7
+        if self.modetype == "TM":
+            return self.beta(k0) / (k0 * C0 * 8.854187818814e-12)
         return Z_VAC
 
     # ---------------------------------------------------------------- fields
@@ -105,7 +127,11 @@ class OraclePort:
         raise ValueError(f"{self.kind} has no mode")
 
     def uinc_local(self, xl, yl, k0):
-        """Incident field of the Robin source term (fem/bc.py:588-589, :711-713)."""
+        """Incident field of the Robin source term (fem/bc.py:466-467, 588-589, 711-713)."""
+        if self.kind == "modal":
+            xg = self.frame.basis @ np.stack([xl, yl, 0 * np.asarray(xl)]) + self.frame.origin[:, None]
+            Eg = self.mode_global(xg[0], xg[1], xg[2])
+            return -2j * self.beta(k0) * (self.frame.inv @ Eg)
         if self.kind == "waveguide":
             return -2j * self.beta(k0) * self.mode_local(xl, yl)
         if self.kind == "lumped":
@@ -114,7 +140,11 @@ class OraclePort:
         return np.zeros((3, np.size(xl)), dtype=np.complex128)
 
     def mode_global(self, xg, yg, zg):
-        """Mode at global points in global components (fem/bc.py:608-618)."""
+        """Mode at global points in global components (fem/bc.py:486-500, 608-618)."""
+        if self.kind == "modal":
+            m = self._mode()
+            pts = np.stack([np.asarray(xg), np.asarray(yg), np.asarray(zg)])
+            return np.sqrt(self.power) * m["norm"] * self.port_field.interpolate_E(m["field"], pts)
         loc = self.frame.to_local_point(xg, yg, zg)
         e = self.mode_local(loc[0], loc[1])
         return self.frame.to_global_vector(*e)
@@ -150,6 +180,8 @@ def from_bc(bc) -> OraclePort:
     if name == "RectangularWaveguide":
         return OraclePort("waveguide", tags, bc.port_number, frame, dims=tuple(bc.dims), power=bc.power,
                           active=bc.active)
+    if name == "ModalPort":
+        return OraclePort("modal", tags, bc.port_number, frame, power=bc.power, active=bc.active)
     if name == "LumpedPort":
         return OraclePort("lumped", tags, bc.port_number, frame, width=bc.width, height=bc.height,
                           direction=np.asarray(bc.direction, dtype=np.float64), Z0=bc.Z0, power=bc.power,

@@ -4,14 +4,18 @@ routines, for the CPU tests that pin the oracle's restatements of them.
 Run in the build container only (the reference source is not available
 elsewhere):  python tests/golden/write_ref_vectors.py /root/reference
 
-The three routines are the ones a restatement could get wrong in a way that
+The routines are the ones a restatement could get wrong in a way that
 brute-force quadrature would flag as a reference quirk rather than a mistake
-(the face-face mass coefficient of fem/mth/tet.py:1036 and the
-non-orthonormal local frame of fem/mth/tri.py:708-721):
+(the face-face mass coefficient of fem/mth/tet.py:1036, the non-orthonormal
+local frame of fem/mth/tri.py:708-721, the 8-digit quadrature table of
+fem/physics/edm/nedeleclegrange2.py:201-204, the longitudinal-field
+expression of fem/mth/tri.py:303):
 
-  * ned2_tet_stiff_mass   fem/mth/tet.py:854-1088
-  * ned2_tri_stiff_force  fem/mth/tri.py:575-695
-  * ned2_tri_stiff        fem/mth/tri.py:697-796
+  * ned2_tet_stiff_mass    fem/mth/tet.py:854-1088
+  * ned2_tri_stiff_force   fem/mth/tri.py:575-695
+  * ned2_tri_stiff         fem/mth/tri.py:697-796
+  * generalized_matrix_GQ  fem/physics/edm/nedeleclegrange2.py:206-374 (port modes)
+  * ned2_tri_interp_full, ned2_tri_interp_curl  fem/mth/tri.py:228-401 (port-mode E, H)
 
 fem/mth/{optimized,tet,tri}.py import only numpy and numba; numba is absent
 here, so a stub module whose ``njit`` returns the function unchanged is put on
@@ -56,6 +60,41 @@ def njit(*args, **kwargs):
 
 jit = njit
 '''
+
+
+_NUMBA_PROGRESS_STUB = '''
+class ProgressBar:
+    def __init__(self, *a, **k):
+        pass
+    def __enter__(self):
+        return self
+    def __exit__(self, *a):
+        return False
+    def update(self, n=1):
+        pass
+ProgressBarType = None
+'''
+
+
+def load_reference_port_elements(ref_root: str):
+    """fem/physics/edm/nedeleclegrange2.py as a module: its generalized_matrix_GQ needs only
+    numpy and fem/mth/optimized.py; the names it imports for its assembly driver (the
+    NedelecLegrange2 field class, numba_progress's ProgressBar) are given empty stand-ins."""
+    stub_dir = os.path.dirname(sys.modules["numba"].__file__)
+    os.makedirs(os.path.join(os.path.dirname(stub_dir), "numba_progress"), exist_ok=True)
+    with open(os.path.join(os.path.dirname(stub_dir), "numba_progress", "__init__.py"), "w") as f:
+        f.write(_NUMBA_PROGRESS_STUB)
+    for name in ("refpkg", "refpkg.physics", "refpkg.physics.edm", "refpkg.elements"):
+        m = types.ModuleType(name)
+        m.__path__ = []
+        sys.modules[name] = m
+    sys.modules["refpkg.physics.edm"].__path__ = [os.path.join(ref_root, "fem", "physics", "edm")]
+    ned = types.ModuleType("refpkg.elements.nedleg2")
+    ned.NedelecLegrange2 = object
+    sys.modules["refpkg.elements.nedleg2"] = ned
+    sys.modules["refpkg.mth"] = sys.modules["refmth"]
+    sys.modules["refpkg.mth.optimized"] = sys.modules["refmth.optimized"]
+    return importlib.import_module("refpkg.physics.edm.nedeleclegrange2")
 
 
 def load_reference_mth(ref_root: str):
@@ -154,6 +193,53 @@ def main(ref_root: str) -> None:
         V.append(verts); E.append(el); G2.append(gamma); B2.append(np.array(Bm))
     out["tris_verts"], out["tris_elen"], out["tris_gamma"], out["tris_B"] = (np.stack(V), np.stack(E), np.array(G2),
                                                                            np.stack(B2))
+    # ---- port-mode element matrices (generalized_matrix_GQ, nedeleclegrange2.py:223-374) and the
+    # 2D field / curl interpolation of a port mode (tri.py:228-401)
+    nl2 = load_reference_port_elements(ref_root)
+    PV, PA, PB, PMs, PMm, PK = [], [], [], [], [], []
+    for i in range(16):
+        scale = [1.0, 1e-3, 5e-3, 2e-2][i % 4]
+        v = rng.normal(size=(2, 3)) * scale + rng.normal(size=(2, 1)) * scale
+        if (v[0, 1] - v[0, 0]) * (v[1, 2] - v[1, 0]) - (v[0, 2] - v[0, 0]) * (v[1, 1] - v[1, 0]) < 0:
+            v[:, [1, 2]] = v[:, [2, 1]]
+        if i % 5 == 4:
+            v[:, [1, 2]] = v[:, [2, 1]]                     # a clockwise triangle too
+        Mm = (rng.uniform(1, 5) * (1 - 0.02j)) * np.eye(3, dtype=np.complex128)
+        Ms = np.eye(3, dtype=np.complex128) / rng.uniform(1, 2)
+        if i % 3 == 2:
+            Mm = np.diag(rng.uniform(1, 4, 3) + 1j * rng.uniform(-0.1, 0, 3)).astype(np.complex128)
+            Ms = np.diag(1 / rng.uniform(1, 2, 3)).astype(np.complex128)
+        k0 = rng.uniform(20, 80) / scale * 1e-3
+        lem = np.array([[0, 1, 0], [1, 2, 2]], dtype=np.int64)
+        A, B = nl2.generalized_matrix_GQ(np.ascontiguousarray(v), lem, Ms, Mm, float(k0))
+        PV.append(v); PA.append(np.array(A)); PB.append(np.array(B)); PMs.append(Ms); PMm.append(Mm); PK.append(k0)
+    out["gq_verts"], out["gq_A"], out["gq_B"] = np.stack(PV), np.stack(PA), np.stack(PB)
+    out["gq_Ms"], out["gq_Mm"], out["gq_k0"] = np.stack(PMs), np.stack(PMm), np.array(PK)
+    # a small planar triangle mesh (2 x 2 squares, 8 triangles) with a random 14-DOF-per-triangle field
+    gx, gy = np.meshgrid(np.linspace(0, 2e-3, 3), np.linspace(0, 1.5e-3, 3))
+    nodes2 = np.stack([gx.ravel(), gy.ravel()]) + rng.normal(size=(2, 9)) * 5e-5 * np.array([[0], [1]])
+    tris = []
+    for r in range(2):
+        for c in range(2):
+            a0, a1, a2, a3 = r * 3 + c, r * 3 + c + 1, (r + 1) * 3 + c, (r + 1) * 3 + c + 1
+            tris += [sorted([a0, a1, a3]), sorted([a0, a3, a2])]
+    tris = np.array(tris, dtype=np.int64).T
+    nf = 40
+    t2f = rng.integers(0, nf, size=(14, tris.shape[1])).astype(np.int64)
+    field = rng.normal(size=nf) + 1j * rng.normal(size=nf)
+    pts = []
+    for t in range(tris.shape[1]):
+        lam = rng.dirichlet(np.ones(3), size=3).T
+        pts.append(nodes2[:, tris[:, t]] @ lam)
+    pts = np.concatenate(pts, axis=1)
+    Ef = tri.ned2_tri_interp_full(pts, field, tris, nodes2, t2f)
+    dia = np.zeros((3, 3, tris.shape[1]), dtype=np.complex128)
+    for t in range(tris.shape[1]):
+        dia[:, :, t] = np.diag(rng.uniform(0.5, 2, 3))
+    beta = 123.4
+    Hf = tri.ned2_tri_interp_curl(pts, field, tris, nodes2, t2f, dia, beta)
+    out["it_nodes"], out["it_tris"], out["it_t2f"], out["it_field"], out["it_pts"] = nodes2, tris, t2f, field, pts
+    out["it_E"], out["it_diadic"], out["it_beta"], out["it_H"] = np.array(Ef), dia, np.array(beta), np.array(Hf)
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT}: " + ", ".join(f"{k}{tuple(v.shape)}" for k, v in out.items()))
 

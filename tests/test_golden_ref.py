@@ -60,3 +60,23 @@ def test_planted_frame_mistake_is_caught(gold):
     # the force variant on orthonormal local coordinates gives the exact 2D block
     B_ortho, _ = fd._tri_stiff_force(loc, gold["tris_gamma"][i], np.zeros((3, 6), complex), fd.gaus_quad_tri4())
     assert _rel(B_ortho, gold["tris_B"][i]) > 1e-6
+
+
+def test_port_element_matrices_match_reference(gold):
+    """oracle.modal.generalized_matrix against generalized_matrix_GQ (nedeleclegrange2.py:206-374),
+    including its 8-digit quadrature table and clockwise triangles."""
+    from oracle import modal as OM
+    for i in range(gold["gq_A"].shape[0]):
+        A, B = OM.generalized_matrix(gold["gq_verts"][i], gold["gq_Ms"][i], gold["gq_Mm"][i], float(gold["gq_k0"][i]))
+        assert _rel(A, gold["gq_A"][i]) <= 1e-12, (i, _rel(A, gold["gq_A"][i]))
+        assert _rel(B, gold["gq_B"][i]) <= 1e-12, (i, _rel(B, gold["gq_B"][i]))
+
+
+def test_port_field_interpolation_matches_reference(gold):
+    """oracle.modal.interp_full / interp_curl against ned2_tri_interp_full / _curl (tri.py:228-401)."""
+    from oracle import modal as OM
+    args = (gold["it_pts"], gold["it_field"], gold["it_tris"], gold["it_nodes"], gold["it_t2f"])
+    E = OM.interp_full(*args)
+    H = OM.interp_curl(*args, gold["it_diadic"], float(gold["it_beta"]))
+    assert _rel(E, gold["it_E"]) <= 1e-12, _rel(E, gold["it_E"])
+    assert _rel(H, gold["it_H"]) <= 1e-12, _rel(H, gold["it_H"])
