@@ -31,10 +31,10 @@ HBM_PEAK_GBS = 8000.0
 # device-to-device copy in the same runs gives.
 ASSEMBLY_TRAFFIC_BYTES = ((52_925 + 1_702_889) * 2 + 425_980 + 8_794_723) * 1024
 # HBM bytes per launch of the sweep's complex64 SpMV (k_spmv_blk<float,16,false>) on the
-# posts box, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this benchmark
-# (--sweep-kernel-only-style run: --spmv-kernel-only --copy-calibration), scaled like the
-# assembly's; None until measured.
-SPMV_TRAFFIC_BYTES = None
+# posts box, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this benchmark
+# (--spmv-kernel-only --copy-calibration): FETCH 602,184 KiB x 2 + WRITE 17,228 KiB x 1,
+# with the factors the 2 GiB copy in the same runs gives (FETCH 1,048,612 KiB, WRITE 2,097,152 KiB).
+SPMV_TRAFFIC_BYTES = (602_184 * 2 + 17_228) * 1024
 FP64_PEAK_TFLOPS = 78.6
 # FP64 flop convention per tet for the dense contraction of the closed form
 FLOP_PER_TET = 15449
