@@ -329,7 +329,7 @@ class OracleProblem:
 
 
 def oracle_eigenmodes(problem: OracleProblem, sigma: float, k: int, spurious_rel: float = 1e-3,
-                      solver: str = "superlu", threads: int | None = None, extra: int = 4):
+                      solver: str = "superlu", threads: int | None = None, extra: int = 4, verbose: bool = False):
     """Deprecated 3D eigen call of the reference (fem/physics/edm/emfreq3d.py:838-848):
     eigs(A, k, M=B, sigma, which='LM') on the PEC-reduced E and B (ARPACK with a
     direct shift-invert factorisation); lambda ~ 0 (gradient) modes removed.
@@ -340,19 +340,33 @@ def oracle_eigenmodes(problem: OracleProblem, sigma: float, k: int, spurious_rel
     A = problem.E[solve_ids][:, solve_ids].tocsc()
     B = problem.B[solve_ids][:, solve_ids].tocsc()
     if solver == "pardiso":
+        import time
         from . import pardiso
         S = (A - sigma * B).tocsr()
         n = S.shape[0]
         real = not np.any(S.data.imag) and not np.any(B.data.imag)
+        t0 = time.perf_counter()
+        calls = [0]
+
+        def tick():
+            calls[0] += 1
+            if verbose and calls[0] % 10 == 0:
+                print(f"  oracle eigen: {calls[0]} solves, {time.perf_counter() - t0:.0f} s", flush=True)
         if real:
             P = pardiso.Pardiso(sp.csr_matrix(S.real), threads=threads)
-            op = lambda x: P.solve(np.real(x)) + 1j * P.solve(np.imag(x)) if np.iscomplexobj(x) else P.solve(x)
+            if verbose:
+                print(f"  oracle eigen: PARDISO factor of n={n} in {time.perf_counter() - t0:.0f} s", flush=True)
+
+            def op(x):
+                tick()
+                return P.solve(np.real(x)) + 1j * P.solve(np.imag(x)) if np.iscomplexobj(x) else P.solve(x)
             A, B = A.real.tocsc(), B.real.tocsc()
         else:
             Sb, _ = pardiso.complex_to_real_block(S, np.zeros(n, dtype=np.complex128))
             P = pardiso.Pardiso(Sb, threads=threads)
 
             def op(x):
+                tick()
                 xr = P.solve(np.hstack([np.real(x), np.imag(x)]))
                 return xr[:n] + 1j * xr[n:]
         OPinv = spla.LinearOperator(S.shape, matvec=op, dtype=np.complex128 if not real else np.float64)

@@ -77,7 +77,9 @@ def c4():
     orc = fd.OracleProblem(phys.mesh, er, ur, phys.boundary_conditions)
     sigma = (2 * np.pi * workloads.CAVITY_TARGET_HZ / C0) ** 2
     t0 = time.perf_counter()
-    w = fd.oracle_eigenmodes(orc, sigma, 10, solver="pardiso", threads=len(os.sched_getaffinity(0)), extra=6)
+    print(f"c4: {phys.mesh.n_tets} tets", flush=True)
+    w = fd.oracle_eigenmodes(orc, sigma, 10, solver="pardiso", threads=len(os.sched_getaffinity(0)), extra=6,
+                             verbose=True)
     print(f"c4 eigenvalues {np.sort(w.real)} ({time.perf_counter() - t0:.1f} s)", flush=True)
     _save(c4_sigma=np.array(sigma), c4_lambda=w)
 

@@ -85,6 +85,9 @@ def test_cavity_workload_modes_match_oracle_fixture(cuda):
     import os
     from emerge_amd import workloads
     gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_fixtures.npz"))
+    if "c4_lambda" not in gold.files:
+        pytest.skip("the oracle's cavity eigenvalues are not in oracle_fixtures.npz "
+                    "(tests/golden/write_oracle_fixtures.py c4 writes them)")
     lam_o = np.sort(gold["c4_lambda"].real)
     phys = workloads.cavity()
     print(f"cavity: {phys.mesh.n_tets} tets", flush=True)
