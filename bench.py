@@ -339,8 +339,9 @@ def eigen_oracle(sigma: float, k: int) -> dict:
 
 def spmv_kernel_roofline(mesh, ports, freq, reps: int = 50):
     """The sweep's dominant kernel, the complex64 entity-blocked SpMV of the
-    mixed-precision GMRES steps (k_spmv_blk<float,16,false>, emg_solver_apply
-    precision 32), timed with HIP events on its stream.  Algorithmic bytes per
+    mixed-precision GMRES steps (k_spmv_blk<float,L,false> with the lanes L the
+    solver's autotuner picked, emg_solver_apply precision 32), timed with HIP
+    events on its stream.  Algorithmic bytes per
     launch (complex64 CSR, int32 indices, one read of x and one write of y):
     12 nnz + 20 N; the blocked format itself moves about 9 nnz + 20 N (one
     column index per four entries)."""
@@ -373,7 +374,9 @@ def spmv_kernel_roofline(mesh, ports, freq, reps: int = 50):
     alg = 12 * nnz + 20 * n
     fmt = 9 * nnz + 20 * n
     ach = alg / (ms * 1e-3) / 1e9
-    return {"kernel": "k_spmv_blk<float,16,false>", "rows": n, "nnz": nnz, "kernel_ms": ms, "alg_bytes": alg,
+    tuning = solver.spmv_tuning()
+    return {"kernel": f"k_spmv_blk<float,{tuning['lanes_c64']},false>", "autotune": tuning, "rows": n, "nnz": nnz,
+            "kernel_ms": ms, "alg_bytes": alg,
             "format_bytes": fmt, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "format_frac": fmt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic": SPMV_TRAFFIC_BYTES}

@@ -202,29 +202,33 @@ __global__ void __launch_bounds__(256) k_spmv_blk(int64_t ne, int64_t half, cons
     }
 }
 
-// Lanes per entity row of the blocked SpMV: EMG_SPMV_BLK_LANES (8, 16 or 32) if
-// set, else 16 (posts-box sweep: 4 lanes 22.1 s, 8 lanes 14.0 s, 16 lanes 12.3 s).
+// Lanes per entity row of the blocked SpMV when nothing was tuned (the public
+// emg_spmv_blocked, or EMG_SPMV_BLK_LANES set): EMG_SPMV_BLK_LANES (4, 8, 16 or
+// 32) if set, else 16.  Solvers tune their own (tune_spmv).
 static int spmv_blk_lanes() {
     static int lanes = 0;
     if (lanes == 0) {
         const char *e = getenv("EMG_SPMV_BLK_LANES");
         const int v = e ? atoi(e) : 16;
-        lanes = (v == 8 || v == 16 || v == 32) ? v : 16;
+        lanes = (v == 4 || v == 8 || v == 16 || v == 32) ? v : 16;
     }
     return lanes;
 }
 
 template <typename R>
 static void launch_spmv_blk(hipStream_t st, int64_t n, int64_t nnz, const int32_t *rowptr, const int32_t *cols,
-                            const R *vals, const uint8_t *mask, const R *x, const R *b, R *y, bool mask_cols = true) {
+                            const R *vals, const uint8_t *mask, const R *x, const R *b, R *y, int L,
+                            bool mask_cols = true) {
     const int64_t ne = n / 2, half = nnz / 2;
-    const int L = spmv_blk_lanes();
+    if (L != 4 && L != 8 && L != 32) L = 16;
     const unsigned grid = (unsigned)((ne * L + 255) / 256);
 #define EMG_BLK(LL, MCV) k_spmv_blk<R, LL, MCV><<<grid, 256, 0, st>>>(ne, half, rowptr, cols, vals, mask, x, b, y)
     if (mask_cols) {
-        if (L == 8) EMG_BLK(8, true); else if (L == 32) EMG_BLK(32, true); else EMG_BLK(16, true);
+        if (L == 4) EMG_BLK(4, true); else if (L == 8) EMG_BLK(8, true); else if (L == 32) EMG_BLK(32, true);
+        else EMG_BLK(16, true);
     } else {
-        if (L == 8) EMG_BLK(8, false); else if (L == 32) EMG_BLK(32, false); else EMG_BLK(16, false);
+        if (L == 4) EMG_BLK(4, false); else if (L == 8) EMG_BLK(8, false); else if (L == 32) EMG_BLK(32, false);
+        else EMG_BLK(16, false);
     }
 #undef EMG_BLK
 }
@@ -670,6 +674,10 @@ struct emg_solver {
     int poly_deg = 1;
     double *poly_c = nullptr;                               // POLY_MAX complex coefficients
     double *pp = nullptr, *pt = nullptr, *py = nullptr, *pu = nullptr;   // work vectors (n complex each)
+    // blocked-SpMV lanes per entity, [0] complex128, [1] complex64: 0 until tuned on
+    // the first matrix (tune_spmv), with the measured milliseconds per variant
+    int lanes[2] = {0, 0};
+    float tune_ms[2][4] = {};
     struct graphs {
         std::vector<hipGraphExec_t> exec;
         const void *vals = nullptr, *inv = nullptr, *mask = nullptr;
@@ -749,7 +757,8 @@ static void apply_prec_poly(emg_solver *s, const R *vals, const R *inv, const R 
     const R *q = x;
     for (int i = 0; i < s->poly_deg - 1; i++) {
         apply_prec<R>(s, inv, q, pu, st);
-        launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, pu, nullptr, pt, sizeof(R) == 8);
+        launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, pu, nullptr, pt, s->lanes[sizeof(R) == 4],
+                           sizeof(R) == 8);
         k_poly_step<R><<<g, RB, 0, st>>>(n, s->poly_c, i, q, pt, pp, py);
         q = pp;
     }
@@ -897,6 +906,71 @@ __global__ void __launch_bounds__(RB) k_scale_tables(int64_t n_ent, const double
     }
 }
 
+// Autotuner of the blocked SpMV: each lanes-per-entity variant (4, 8, 16, 32) runs
+// once untimed and five times between HIP events on the solver's first matrix, in
+// the precision the solver's steps use; the fastest is kept for the solver's life
+// (the sparsity pattern fixes the row lengths that decide it).  Vectors: zeroed
+// solver work buffers.  EMG_SPMV_BLK_LANES, when set, overrides the search.
+static const int TUNE_LANES[4] = {4, 8, 16, 32};
+
+template <typename R>
+static int tune_spmv_one(emg_solver *s, const R *vals, int slot, hipStream_t st) {
+    hipEvent_t ev[2];
+    EMG_HIP(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        emg::set_error("tune_spmv: hipEventCreate failed");
+        return -2;
+    }
+    hipError_t e = hipSuccess;
+    float best = 3.4e38f;
+    int bl = 16;
+    for (int i = 0; i < 4 && e == hipSuccess; i++) {
+        const int L = TUNE_LANES[i];
+        auto run = [&]() {
+            launch_spmv_blk<R>(st, s->n, s->nnz, s->rowptr, s->cols, vals, s->mask, reinterpret_cast<const R *>(s->z),
+                               nullptr, reinterpret_cast<R *>(s->pp), L, sizeof(R) == 8);
+        };
+        run();
+        e = hipEventRecord(ev[0], st);
+        for (int r = 0; r < 5; r++) run();
+        if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        ms /= 5.f;
+        s->tune_ms[slot][i] = ms;
+        if (e == hipSuccess && ms < best) { best = ms; bl = L; }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    EMG_HIP(e);
+    s->lanes[slot] = bl;
+    return 0;
+}
+
+static int tune_spmv(emg_solver *s, hipStream_t st) {
+    if (getenv("EMG_SPMV_BLK_LANES") != nullptr) {
+        s->lanes[0] = s->lanes[1] = spmv_blk_lanes();
+        return 0;
+    }
+    EMG_HIP(hipMemsetAsync(s->z, 0, s->n * 16, st));
+    int rc = tune_spmv_one<double>(s, s->vals, 0, st);
+    if (rc) return rc;
+    if (s->precision == 32) return tune_spmv_one<float>(s, s->vals32, 1, st);
+    s->lanes[1] = s->lanes[0];
+    return 0;
+}
+
+extern "C" int emg_solver_spmv_tuning(emg_solver *s, int *h_lanes, float *h_ms) {
+    EMG_CHECK(s && h_lanes && h_ms, "emg_solver_spmv_tuning: null argument");
+    for (int p = 0; p < 2; p++) {
+        h_lanes[p] = s->lanes[p];
+        for (int i = 0; i < 4; i++) h_ms[4 * p + i] = s->tune_ms[p][i];
+    }
+    return 0;
+}
+
 extern "C" int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const uint8_t *d_mask, void *stream) {
     EMG_CHECK(s && d_vals, "emg_solver_set_matrix: null argument");
     hipStream_t st = (hipStream_t)stream;
@@ -916,6 +990,10 @@ extern "C" int emg_solver_set_matrix(emg_solver *s, const double *d_vals, const 
                                                                s->np2 > 0 ? s->w_p : nullptr, s->dsinv64, s->dsinv32);
     }
     EMG_LAUNCH_CHECK();
+    if (s->lanes[0] == 0) {
+        const int rc = tune_spmv(s, st);
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -991,7 +1069,7 @@ extern "C" int emg_solver_arnoldi_probe(emg_solver *s, int d, const double *d_v0
         const int nv = j + 1, nbm = mdot_blocks(n, nv);
         const dim3 grid(nbm, (nv + DT - 1) / DT);
         apply_prec<double>(s, s->inv64, vj, z, st);
-        launch_spmv_blk<double>(st, n, s->nnz, s->rowptr, s->cols, s->vals, s->mask, z, nullptr, w);
+        launch_spmv_blk<double>(st, n, s->nnz, s->rowptr, s->cols, s->vals, s->mask, z, nullptr, w, s->lanes[0]);
         double wn = 0;
         for (int pass = 0; pass < 2; pass++) {
             k_mdot<double><<<grid, RB, 0, st>>>(n, nv, V, n, w, s->partial, nullptr);
@@ -1178,7 +1256,8 @@ static void enqueue_step(emg_solver *s, const R *vals, const R *inv, int j, hipS
     const dim3 grid(nbm, (nv + DT - 1) / DT);
     apply_prec_poly<R>(s, vals, inv, vj, z, st);
     // the complex64 copy has its masked columns zeroed (k_to_c64_masked)
-    launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn, sizeof(R) == 8);
+    launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, z, nullptr, vn, s->lanes[sizeof(R) == 4],
+                       sizeof(R) == 8);
     k_mdot<R><<<grid, RB, 0, st>>>(n, nv, V, n, vn, s->partial, nullptr);
     k_mdot_finish<<<nv, 256, 0, st>>>(nv, nbm, s->partial, h1, 0, nullptr);
     k_maxpy_norm<R><<<nb, RB, 0, st>>>(n, nv, V, n, h1, vn, s->pn, nullptr);
@@ -1254,7 +1333,8 @@ static int gmres_core(emg_solver *s, const R *vals, const R *inv, const R *rhs, 
     double resid = 1.0;
     while (true) {
         // r = rhs - K u  ->  v0 = r / |r|
-        launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, u, rhs, w, sizeof(R) == 8);
+        launch_spmv_blk<R>(st, n, s->nnz, s->rowptr, s->cols, vals, s->mask, u, rhs, w, s->lanes[sizeof(R) == 4],
+                           sizeof(R) == 8);
         k_norm2<R><<<nb, RB, 0, st>>>(n, w, s->partial);
         k_scale_by_norm<R><<<nb, RB, 0, st>>>(n, s->partial, nb, w, V, s->nrm);
         double beta;
@@ -1316,7 +1396,7 @@ extern "C" int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, d
     memset(info, 0, sizeof(*info));
     auto true_resid = [&](double *out_rel, double *bn) -> int {
         // r = b - K x in complex128 (masked)
-        launch_spmv_blk<double>(st, n, s->nnz, s->rowptr, s->cols, s->vals, s->mask, d_x, d_b, s->r);
+        launch_spmv_blk<double>(st, n, s->nnz, s->rowptr, s->cols, s->vals, s->mask, d_x, d_b, s->r, s->lanes[0]);
         k_norm2<double><<<nb, RB, 0, st>>>(n, s->r, s->partial);
         k_norm2<double><<<nb, RB, 0, st>>>(n, d_b, s->partial + 1024);
         std::vector<double> p(2048);
@@ -1424,7 +1504,8 @@ extern "C" int emg_spmv_blocked(int64_t n, int64_t nnz, const int32_t *d_rowptr,
                                 void *stream) {
     EMG_CHECK(d_rowptr && d_cols && d_vals && d_x && d_y, "emg_spmv_blocked: null argument");
     EMG_CHECK(n > 0 && n % 2 == 0 && nnz % 2 == 0, "emg_spmv_blocked: n and nnz must be even");
-    launch_spmv_blk<double>((hipStream_t)stream, n, nnz, d_rowptr, d_cols, d_vals, d_mask, d_x, nullptr, d_y);
+    launch_spmv_blk<double>((hipStream_t)stream, n, nnz, d_rowptr, d_cols, d_vals, d_mask, d_x, nullptr, d_y,
+                            spmv_blk_lanes());
     EMG_LAUNCH_CHECK();
     return 0;
 }
@@ -1438,11 +1519,11 @@ extern "C" int emg_solver_apply(emg_solver *s, int precision, const void *d_x, v
     hipStream_t st = (hipStream_t)stream;
     if (precision == 64) {
         launch_spmv_blk<double>(st, s->n, s->nnz, s->rowptr, s->cols, s->vals, s->mask,
-                                static_cast<const double *>(d_x), nullptr, static_cast<double *>(d_y));
+                                static_cast<const double *>(d_x), nullptr, static_cast<double *>(d_y), s->lanes[0]);
     } else {
         EMG_CHECK(precision == 32 && s->precision == 32, "emg_solver_apply: precision 32 needs a precision-32 solver");
         launch_spmv_blk<float>(st, s->n, s->nnz, s->rowptr, s->cols, s->vals32, s->mask,
-                               static_cast<const float *>(d_x), nullptr, static_cast<float *>(d_y), false);
+                               static_cast<const float *>(d_x), nullptr, static_cast<float *>(d_y), s->lanes[1], false);
     }
     EMG_LAUNCH_CHECK();
     return 0;

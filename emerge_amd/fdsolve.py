@@ -277,6 +277,18 @@ class GPUSolver:
     learned = None          # LearnedSmoother whose weights scale the smoother at each frequency
     POLY_STEP_CAP = 3000    # GMRES steps allowed with the polynomial before the plain smoother takes over
 
+    def spmv_tuning(self) -> dict:
+        """The autotuned SpMV shape of this solver (chosen on its first matrix): lanes per
+        entity for complex128 and complex64 steps and the measured ms per launch of each
+        candidate (4, 8, 16, 32 lanes); lanes 0 before the first set_matrix."""
+        lanes = (ctypes.c_int * 2)()
+        ms = (ctypes.c_float * 8)()
+        _lib.check(self.L.emg_solver_spmv_tuning(self.h, lanes, ms), "emg_solver_spmv_tuning")
+        cand = (4, 8, 16, 32)
+        return {"lanes_c128": int(lanes[0]), "lanes_c64": int(lanes[1]),
+                "ms_c128": dict(zip(cand, [float(v) for v in ms[0:4]])),
+                "ms_c64": dict(zip(cand, [float(v) for v in ms[4:8]]))}
+
     def set_weights(self, we, wp):
         """Learned smoother weights (complex128 device tensors or None) for the next set_matrix."""
         self._w = (we, wp)

@@ -141,6 +141,14 @@ int emg_solver_arnoldi_probe(emg_solver *s, int d, const double *d_v0, double *h
  * solver's complex64 copy (x, y complex64).  No counterpart in the reference
  * (scipy's CSR product inside its solvers); exported to time the product. */
 int emg_solver_apply(emg_solver *s, int precision, const void *d_x, void *d_y, void *stream);
+
+/* The solver's autotuned SpMV shape: on its first emg_solver_set_matrix the solver times
+ * the blocked SpMV with 4, 8, 16 and 32 lanes per entity (HIP events, 5 launches each, in
+ * complex128 and, for precision 32, complex64) and keeps the fastest.  h_lanes[2]: the
+ * chosen lanes (complex128, complex64); h_ms[8]: the measured milliseconds per launch
+ * (4 per precision, in the order 4, 8, 16, 32).  EMG_SPMV_BLK_LANES overrides the search.
+ * No reference counterpart (PARDISO has no shape to tune).                              */
+int emg_solver_spmv_tuning(emg_solver *s, int *h_lanes, float *h_ms);
 /* d_x: initial guess in, solution out (n complex128). Synchronises `stream`. */
 int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, double tol, int maxiter, int max_refine,
                      emg_solve_info *info, void *stream);

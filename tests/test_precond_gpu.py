@@ -109,6 +109,49 @@ def test_blocked_spmv_matches_csr_spmv(cuda):
         assert err < 1e-14
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_spmv_autotuner_picks_the_fastest_shape(cuda, precision):
+    """The solver's SpMV autotuner: nothing tuned before the first matrix; then every
+    candidate (4, 8, 16, 32 lanes per entity) has a measured time, the kept shape is the
+    fastest measured, and the tuned product equals the scalar CSR SpMV."""
+    import torch
+    from emerge_amd import _lib
+    from emerge_amd.fdsolve import GPUSolver
+    mesh, phys = _posts(10, 5, 2)
+    prob = phys.problem()
+    s = GPUSolver(prob, precision=precision)
+    assert s.spmv_tuning()["lanes_c128"] == 0
+    K = prob.system(3e9)
+    s.set_matrix(K)
+    t = s.spmv_tuning()
+    print("spmv tuning", t, flush=True)
+    for key in ("c128",) + (("c64",) if precision == 32 else ()):
+        ms = t["ms_" + key]
+        assert all(v > 0 for v in ms.values())
+        assert t["lanes_" + key] == min(ms, key=ms.get)
+    L = _lib.load()
+    g = torch.Generator(device=prob.dev)
+    g.manual_seed(7)
+    x = torch.view_as_complex(torch.randn((prob.N, 2), generator=g, dtype=torch.float64, device=prob.dev)).contiguous()
+    y1 = torch.empty_like(x)
+    _lib.check(L.emg_spmv(prob.N, _lib.ptr(prob.pat.rowptr), _lib.ptr(prob.pat.cols), _lib.ptr(K), _lib.ptr(prob.mask),
+                          _lib.ptr(x), _lib.ptr(y1), _lib.stream_ptr()), "emg_spmv")
+    if precision == 64:
+        y2 = torch.empty_like(x)
+        _lib.check(L.emg_solver_apply(s.h, 64, _lib.ptr(x), _lib.ptr(y2), _lib.stream_ptr()), "emg_solver_apply")
+        tol = 1e-14
+    else:
+        x32 = x.to(torch.complex64).contiguous()
+        y32 = torch.empty_like(x32)
+        _lib.check(L.emg_solver_apply(s.h, 32, _lib.ptr(x32), _lib.ptr(y32), _lib.stream_ptr()), "emg_solver_apply")
+        y2 = y32.to(torch.complex128)
+        tol = 1e-5
+    torch.cuda.synchronize()
+    err = float((y1 - y2).abs().max() / y1.abs().max())
+    print("tuned vs csr", precision, err, flush=True)
+    assert err < tol
+
+
 def test_solver_rejects_csr_off_the_entity_pattern(cuda):
     """A PEC-reduced or re-sorted CSR breaks the blocked SpMV's layout: emg_solver_create refuses it."""
     import ctypes
