@@ -382,6 +382,63 @@ def spmv_kernel_roofline(mesh, ports, freq, reps: int = 50):
             "traffic": SPMV_TRAFFIC_BYTES}
 
 
+def learned_bound(name: str, log) -> dict:
+    """What the learned smoother can at best gain under its training loss: the smoother
+    weights optimised directly on the residuals of the very frequency then solved
+    (LearnedSmoother.optimal_weights, every entity and node weight free, no network), then
+    GMRES steps per port with and without them (emg_solver_solve, no fallback), in the
+    sweep's precision (complex64 steps in FP64 refinement), with and without the degree-6
+    polynomial.  posts: the posts box at 2.5 GHz, bounded (mode 2) and unbounded complex
+    (mode 0) weights, at most 8,000 steps; patch: the patch antenna at 2.45 GHz, bounded
+    weights, degree 1, at most 40,000 steps."""
+    import ctypes
+    import torch
+    from emerge_amd import _lib, workloads
+    from emerge_amd.fdsolve import GPUSolver
+    from emerge_amd.learned import LearnedSmoother
+    if name == "posts":
+        phys, freqs, modes, degrees, maxit = workloads.posts(), (2.5e9,), (2, 0), (1, 6), 8000
+    else:
+        phys, freqs, modes, degrees, maxit = workloads.patch(), (2.45e9,), (2,), (1,), 40000
+    prob = phys.problem()
+    L = _lib.load()
+    out = {"workload": name, "rows": prob.N, "max_steps": maxit, "cases": []}
+    for f in freqs:
+        K = prob.system(f)
+        for mode in modes:
+            t0 = time.perf_counter()
+            sc = GPUSolver(prob, precision=64, poly_degree=1)
+            net = LearnedSmoother(prob, mode=mode)
+            prep = net.prepare(sc, net.collect(sc, [f], tol=1e-8))
+            del sc
+            we, wp, hist = net.optimal_weights(prep, steps=200, lr=0.02, log=log)
+            log(f"learned bound {name} f={f / 1e9:.2f} GHz mode {mode}: loss {hist[0]:.4g} -> {hist[-1]:.4g} "
+                f"({time.perf_counter() - t0:.0f} s)")
+            rows = []
+            for d in degrees:
+                for wname, w in (("plain", (None, None)), ("optimal", (we, wp))):
+                    s = GPUSolver(prob, precision=32, poly_degree=d)
+                    s.POLY_STEP_CAP = maxit
+                    its, conv, res = [], [], []
+                    for port in prob.ports:
+                        s.set_weights(*w)
+                        s.set_matrix(K)
+                        b = prob.rhs(port, f)
+                        x = torch.zeros_like(b)
+                        info = _lib.SolveInfo()
+                        _lib.check(L.emg_solver_solve(s.h, _lib.ptr(b), _lib.ptr(x), 1e-8, maxit, 0,
+                                                      ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
+                        its.append(int(info.iterations))
+                        conv.append(bool(info.converged))
+                        res.append(float(info.residual))
+                    del s
+                    rows.append({"poly_degree": d, "weights": wname, "steps": its, "converged": conv, "residual": res})
+                    log(f"  degree {d} {wname}: steps {its} converged {conv} residual {[f'{r:.2e}' for r in res]}")
+            out["cases"].append({"freq_hz": f, "weight_mode": "bounded [0.61, 1.65]" if mode == 2 else "complex 1 + o",
+                                 "loss_first": hist[0], "loss_last": hist[-1], "solves": rows})
+    return out
+
+
 def cpu_sweep_child(name: str, freq: float) -> dict:
     """Run cpu_sweep_baseline in a child process that never touches the GPU (MKL's
     threads and allocator then run as they do in a plain host program)."""
@@ -488,6 +545,8 @@ def main():
                     help="measure only the sweep's complex64 SpMV kernel (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 2 GiB device-to-device copy (reference for PMC byte counters)")
+    ap.add_argument("--learned-bound", default=None, choices=["posts", "patch"],
+                    help="only measure what the learned smoother's loss-optimal weights do to GMRES")
     ap.add_argument("--cpu-sweep-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-sweep-freq", type=float, default=3e9, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -505,6 +564,11 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the frequency-sharded "
               "multi-GPU design is described in the docs but not run here", file=sys.stderr)
         sys.exit(2)
+
+    if args.learned_bound:
+        res = learned_bound(args.learned_bound, lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps({"learned_bound": res}), flush=True)
+        return
 
     import torch
     from emerge_amd import meshgen

@@ -278,11 +278,13 @@ class LearnedSmoother:
             out.append((f, K, R, U, T, scale))
         return out
 
-    def loss_and_grad(self, f, K, R, U, T, scale, accumulate: bool = True):
-        """J at frequency f and its gradient into self.grads (network forward + backward)."""
+    def loss_and_grad(self, f, K, R, U, T, scale, accumulate: bool = True, readouts=None):
+        """J at frequency f and its gradient into self.grads (network forward + backward).
+        With readouts = (Ov, Oe, Of) given, the network is bypassed: J of those readouts
+        and their gradient (dOv, dOe, dOf) are returned instead."""
         torch, L, P, st = self.torch, self.L, _lib.ptr, _lib.stream_ptr()
         G = self.G
-        Ov, Oe, Of = self.forward(self.features(f))
+        Ov, Oe, Of = self.forward(self.features(f)) if readouts is None else readouts
         we, wp = self.readouts_to_weights(Ov, Oe, Of)
         n_ent = we.shape[0]
         W = torch.cat([we, we])
@@ -313,15 +315,46 @@ class LearnedSmoother:
                    "emg_gnn_weight_grad")
         _lib.check(L.emg_gnn_weight_grad(G.np2, ns, G.np2, 0, P(T), P(Gtg), P(scale), P(dnode), st),
                    "emg_gnn_weight_grad")
-        if not accumulate:
-            for v in self.grads.values():
-                v.zero_()
         Oent, Onode = self._O
         dOent, dOnode = torch.empty_like(dent), torch.empty_like(dnode)
         _lib.check(L.emg_gnn_weights(n_ent, self.mode, P(Oent), None, P(dent), P(dOent), st), "emg_gnn_weights")
         _lib.check(L.emg_gnn_weights(G.np2, self.mode, P(Onode), None, P(dnode), P(dOnode), st), "emg_gnn_weights")
+        if readouts is not None:
+            return J, self.weights_grad_to_readouts(dOent, dOnode)
+        if not accumulate:
+            for v in self.grads.values():
+                v.zero_()
         self.backward(*self.weights_grad_to_readouts(dOent, dOnode))
         return J
+
+    def optimal_weights(self, prepared, steps: int = 200, lr: float = 0.02, log=None):
+        """The weights that minimise the training loss directly: every entity and node
+        readout a free parameter (no network, no features), Adam on the same J and
+        gradient.  What any network trained on this loss can at best produce, so its
+        GMRES iterations bound what the network can gain.  Returns (we, wp, history)."""
+        torch = self.torch
+        O = [torch.zeros((self.nv, 2), dtype=torch.float32, device=self.dev),
+             torch.zeros((self.ne, 4), dtype=torch.float32, device=self.dev),
+             torch.zeros((self.nf, 2), dtype=torch.float32, device=self.dev)]
+        prm = [torch.nn.Parameter(o) for o in O]
+        opt = torch.optim.Adam(prm, lr=lr)
+        hist = []
+        for it in range(steps):
+            J = 0.0
+            gsum = [torch.zeros_like(o) for o in O]
+            for item in prepared:
+                j, g = self.loss_and_grad(*item, readouts=tuple(p.data for p in prm))
+                J += j
+                for a, b in zip(gsum, g):
+                    a += b
+            for p_, g in zip(prm, gsum):
+                p_.grad = g
+            opt.step()
+            hist.append(J)
+            if log is not None and (it % 50 == 0 or it == steps - 1):
+                log(f"optimal weights step {it}: loss {J:.6g}")
+        we, wp = self.readouts_to_weights(*(p.data for p in prm))
+        return we, wp, hist
 
     def train(self, prepared, steps: int = 150, lr: float = 2e-3, log=None):
         torch = self.torch
