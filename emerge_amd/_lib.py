@@ -46,6 +46,10 @@ SIGNATURES: dict[str, tuple] = {
     "emg_farfield": (ctypes.c_int, [I64, P, P, P, P, I64, P, P, F64, P, P, P]),
     "emg_locate_points": (ctypes.c_int, [I64, P, P, P, P, P, P, P, P, P]),
     "emg_solver_spmv_tuning": (ctypes.c_int, [P, P, P]),
+    "emg_topology_build": (ctypes.c_int, [I64, I64, P, P, P, P]),
+    "emg_topology_sizes": (ctypes.c_int, [P, P, P]),
+    "emg_topology_get": (ctypes.c_int, [P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "emg_topology_destroy": (None, [P]),
     "emg_port_assemble": (ctypes.c_int, [I64, P, P, P, F64, ctypes.c_int, P, I64, P, P, P, P, P]),
     "emg_port_field": (ctypes.c_int, [I64, P, I64, P, I64, P, P, I64, P, F64, P, P, F64, F64, P, P, P]),
     "emg_port_factor": (ctypes.c_int, [I64, P, P, F64, F64, P, P, P]),

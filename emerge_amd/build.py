@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemerge_amd.so")
-SOURCES = ["emg_assembly.hip", "emg_krylov.hip", "emg_boundary.hip", "emg_sweep.hip", "emg_gnn.hip", "emg_perm.hip", "emg_farfield.hip", "emg_port.hip"]
+SOURCES = ["emg_assembly.hip", "emg_krylov.hip", "emg_boundary.hip", "emg_sweep.hip", "emg_gnn.hip", "emg_perm.hip", "emg_farfield.hip", "emg_port.hip", "emg_topology.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -18,6 +18,8 @@ Differences, all deliberate:
 """
 from __future__ import annotations
 
+import time
+
 import numpy as np
 
 CANON_EDGES = ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))
@@ -38,10 +40,12 @@ class Mesh3D:
         tet_tags: optional (nt,) int volume tag per tet.
         face_tags: optional dict {tag: (3, k) int vertex triples} of tagged
             boundary/interface triangles (the stand-in for gmsh surface tags).
+        topology: "host" (numpy) or "device" (the GPU build of emg_topology_build,
+            identical arrays; ``topology_seconds`` records either's time).
     """
 
     def __init__(self, nodes: np.ndarray, tets: np.ndarray, tet_tags: np.ndarray | None = None,
-                 face_tags: dict[int, np.ndarray] | None = None):
+                 face_tags: dict[int, np.ndarray] | None = None, topology: str = "host"):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
         self.tets = np.ascontiguousarray(tets, dtype=np.int64)
         nv, nt = self.nodes.shape[1], self.tets.shape[1]
@@ -51,11 +55,40 @@ class Mesh3D:
             raise ValueError("tet vertex id out of range")
         self.tet_tags = np.zeros(nt, dtype=np.int64) if tet_tags is None else np.asarray(tet_tags, dtype=np.int64)
 
+        t0 = time.perf_counter()
+        self._nv = nv
+        if topology == "device":
+            self._topology_device()
+        elif topology == "host":
+            self._topology_host()
+        else:
+            raise ValueError(f"topology is 'host' or 'device', not {topology!r}")
+        self.topology_seconds = time.perf_counter() - t0
+        t = self.tris
+        T = self.tets
+        # fem/mesh3d.py:249, 273, 351-353
+        self.centers = self.nodes[:, T].mean(axis=1)
+        self.tri_centers = self.nodes[:, t].mean(axis=1)
+        self.edge_centers = self.nodes[:, self.edges].mean(axis=1)
+        if topology == "host":
+            self.edge_lengths = np.sqrt(np.sum((self.nodes[:, self.edges[0]] - self.nodes[:, self.edges[1]]) ** 2, axis=0))
+        a = self.nodes[:, t[1]] - self.nodes[:, t[0]]
+        b = self.nodes[:, t[2]] - self.nodes[:, t[0]]
+        self.areas = 0.5 * np.linalg.norm(np.cross(a.T, b.T), axis=1)
+
+        self.face_tags: dict[int, np.ndarray] = {}
+        for tag, tri in (face_tags or {}).items():
+            tri = np.asarray(tri, dtype=np.int64).reshape(3, -1)
+            self.face_tags[int(tag)] = np.unique(self.get_tri(tri[0], tri[1], tri[2]))
+        self.defined = True
+
+    def _topology_host(self):
+        """Mesh3D.update's topology (fem/mesh3d.py:252-336) with numpy sorts and uniques."""
+        nv, nt = self._nv, self.tets.shape[1]
         canon = np.sort(self.tets, axis=0)                                    # (4, nt)
         self.tets_canonical = np.ascontiguousarray(canon.T.astype(np.int32))  # (nt, 4)
         e_all = np.stack([np.stack([canon[a], canon[b]]) for a, b in CANON_EDGES], axis=1)      # (2, 6, nt)
         f_all = np.stack([np.stack([canon[a], canon[b], canon[c]]) for a, b, c in CANON_FACES], axis=1)  # (3, 4, nt)
-        self._nv = nv
         ekeys = e_all[0] * nv + e_all[1]                                      # (6, nt)
         uek, einv = np.unique(ekeys.ravel(), return_inverse=True)
         self.edges = np.stack([uek // nv, uek % nv]).astype(np.int64)
@@ -93,20 +126,19 @@ class Mesh3D:
         t = self.tris
         self.tri_to_edge = np.stack([self.get_edge(t[0], t[1]), self.get_edge(t[1], t[2]), self.get_edge(t[0], t[2])])
         self.tri_to_edge_sign = np.stack([np.ones(nf, np.int64), np.ones(nf, np.int64), -np.ones(nf, np.int64)])
-        # fem/mesh3d.py:249, 273, 351-353
-        self.centers = self.nodes[:, T].mean(axis=1)
-        self.tri_centers = self.nodes[:, t].mean(axis=1)
-        self.edge_centers = self.nodes[:, self.edges].mean(axis=1)
-        self.edge_lengths = np.sqrt(np.sum((self.nodes[:, self.edges[0]] - self.nodes[:, self.edges[1]]) ** 2, axis=0))
-        a = self.nodes[:, t[1]] - self.nodes[:, t[0]]
-        b = self.nodes[:, t[2]] - self.nodes[:, t[0]]
-        self.areas = 0.5 * np.linalg.norm(np.cross(a.T, b.T), axis=1)
 
-        self.face_tags: dict[int, np.ndarray] = {}
-        for tag, tri in (face_tags or {}).items():
-            tri = np.asarray(tri, dtype=np.int64).reshape(3, -1)
-            self.face_tags[int(tag)] = np.unique(self.get_tri(tri[0], tri[1], tri[2]))
-        self.defined = True
+    def _topology_device(self):
+        """The same arrays from the GPU (emg_topology_build): one stable radix sort of edge
+        keys and one of face keys, then binary searches; copied back to host numpy."""
+        from .topology import device_topology
+        out = device_topology(self.nodes, self.tets)
+        for k, v in out.items():
+            setattr(self, k, v)
+        nv = self._nv
+        self._ekeys = self.edges[0] * nv + self.edges[1]
+        self._fkeys = self._face_key(self.tris[0], self.tris[1], self.tris[2])
+        nf = self.tris.shape[1]
+        self.tri_to_edge_sign = np.stack([np.ones(nf, np.int64), np.ones(nf, np.int64), -np.ones(nf, np.int64)])
 
     # ------------------------------------------------------------ keys
     def _face_key(self, a, b, c):

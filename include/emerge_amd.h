@@ -235,6 +235,31 @@ int emg_farfield(int64_t ns, const double *d_pts, const double *d_wn, const doub
                  int64_t ndir, const double *d_theta, const double *d_phi, double k0, double *d_Eout,
                  double *d_Hout, void *stream);
 
+/* ---- mesh topology on the GPU (fem/mesh3d.py:224-355, fem/elements/nedelec2.py:32-70) ---- */
+typedef struct emg_topology emg_topology;
+
+/* Topology of nt tets over nv vertices (d_tets: 4 x nt int64, row k = the tets' k-th vertex, any
+ * vertex order; d_nodes: 3 x nv float64 or NULL, for the edge lengths): unique edges and faces
+ * numbered in lexicographic order of their sorted vertex ids, by stable radix sorts of 64-bit
+ * keys.  Replaces Mesh3D.update's set/dict loops (mesh3d.py:252-336) and the entity numbering
+ * of Nedelec2 (nedelec2.py:46-62).  Fails (-1) on a vertex id out of range or repeated in a
+ * tet, and on a triangle shared by more than two tets; nv < 2^21.                        */
+int emg_topology_build(int64_t nv, int64_t nt, const int64_t *d_tets, const double *d_nodes, emg_topology **out,
+                       void *stream);
+int emg_topology_sizes(const emg_topology *h, int64_t *ne, int64_t *nf);
+/* Copies (each destination optional, NULL to skip): edges 2 x ne, tris 3 x nf (int64, sorted
+ * vertex ids); tets_canonical nt x 4 and tet_ent nt x 10 (int32: 6 edges in the local order
+ * (0,1)(0,2)(0,3)(1,2)(1,3)(2,3), then ne + 4 faces (0,1,2)(0,2,3)(0,1,3)(1,2,3) of the sorted
+ * tet); tet_to_edge and its sign 6 x nt and tet_to_tri and its sign 4 x nt in the reference's
+ * local orders on the tet's own vertex order (mesh3d.py:292-305, the sign of _tri_ordering as
+ * written); tri_to_tet 2 x nf (tets in ascending order, -1 padded); tri_to_edge 3 x nf (edges
+ * (i1,i2), (i2,i3), (i1,i3)); edge_lengths ne float64.                                      */
+int emg_topology_get(const emg_topology *h, int64_t *d_edges, int64_t *d_tris, int32_t *d_tets_canonical,
+                     int32_t *d_tet_ent, int64_t *d_tet_to_edge, int64_t *d_tet_to_edge_sign, int64_t *d_tet_to_tri,
+                     int64_t *d_tet_to_tri_sign, int64_t *d_tri_to_tet, int64_t *d_tri_to_edge,
+                     double *d_edge_lengths, void *stream);
+void emg_topology_destroy(emg_topology *h);
+
 /* ---- port-face modal analysis (fem/physics/edm/emfreq3d.py:201-364) ---- */
 #define EMG_PORT_FACE_BLOCK 1 /* fill the face-face entries of Dtt, which the reference's
                                  generalized_matrix_GQ leaves zero (nedeleclegrange2.py:350-358) */
