@@ -145,6 +145,34 @@ class GPUPattern:
                                       _lib.ptr(nu), flags, _lib.ptr(E), _lib.ptr(B), _lib.stream_ptr()),
                    "emg_assemble_iso")
 
+    def assemble_iso_batched(self, nodes, eps, nu, flags: int = 0):
+        """E and B of several meshes that share this pattern's tets, in one fused launch pair.
+
+        nodes: (M, nv, 3) float64 coordinates per mesh (device tensor or numpy), eps, nu:
+        (M, nt) complex128 eps_r and 1/mu_r per tet.  Returns device tensors E, B of shape
+        (M, nnz); mesh b's values equal assemble_iso_device's on that mesh."""
+        import torch
+        L = _lib.load()
+        dev = self.device
+        nodes = torch.as_tensor(nodes, dtype=torch.float64, device=dev).contiguous()
+        eps = torch.as_tensor(eps, dtype=torch.complex128, device=dev).contiguous()
+        nu = torch.as_tensor(nu, dtype=torch.complex128, device=dev).contiguous()
+        M, nv, nt = nodes.shape[0], self.mesh.n_nodes, self.mesh.n_tets
+        if nodes.shape != (M, nv, 3) or eps.shape != (M, nt) or nu.shape != (M, nt):
+            raise ValueError(f"batch shapes: nodes {tuple(nodes.shape)}, eps {tuple(eps.shape)}, nu {tuple(nu.shape)}; "
+                             f"expected ({M}, {nv}, 3), ({M}, {nt}), ({M}, {nt})")
+        tets = self.tets
+        if flags & REFERENCE_GEOMETRY:
+            if getattr(self, "tets_ref", None) is None:
+                self.tets_ref = torch.from_numpy(np.ascontiguousarray(self.mesh.tets.T.astype(np.int32))).to(dev)
+            tets = self.tets_ref
+        E = torch.empty((M, self.nnz), dtype=torch.complex128, device=dev)
+        B = torch.empty_like(E)
+        _lib.check(L.emg_assemble_iso_batched(self._handle, M, nv, _lib.ptr(nodes), _lib.ptr(tets), _lib.ptr(eps),
+                                              _lib.ptr(nu), flags, _lib.ptr(E), _lib.ptr(B), _lib.stream_ptr()),
+                   "emg_assemble_iso_batched")
+        return E, B
+
     def to_scipy(self, vals) -> sp.csr_matrix:
         """Device values -> host scipy CSR with this pattern."""
         return sp.csr_matrix((vals.cpu().numpy(), self.cols.cpu().numpy(), self.rowptr.cpu().numpy()),

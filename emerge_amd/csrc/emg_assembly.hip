@@ -68,6 +68,8 @@ struct emg_pattern {
     uint16_t *inc_local = nullptr;   // n_inc: (local tet << 8) | (s << 4) | s'
     uint8_t *chunk_perm = nullptr;   // nchunk * CHUNK: chunk-local entries by descending incidence count
     double *tet_rec = nullptr;       // nt * REC: per-tet Gram records (fused-path workspace)
+    double *tet_rec_batch = nullptr; // rec_batch_cap * nt * REC: the records of a batch of meshes
+    int rec_batch_cap = 0;
     uint8_t *recipes = nullptr;      // 100 * RECIPE_BYTES
 };
 
@@ -325,6 +327,7 @@ extern "C" void emg_pattern_destroy(emg_pattern *pat) {
     hfree(pat->chunk_perm);
     hfree(pat->chunk_row0);
     hfree(pat->tet_rec);
+    hfree(pat->tet_rec_batch);
     hfree(pat->recipes);
     delete pat;
 }
@@ -1011,12 +1014,19 @@ __device__ __attribute__((noinline)) void reference_bcd(const V3 (&Q)[4], double
 // (times 6V) are cross products of edge vectors relative to vertex 0.
 // REF_GEOM = true: tets are in the reference's vertex order; the gradients and volume
 // come from reference_bcd and are re-indexed to the canonical (sorted) order.
+// blockIdx.y selects the mesh of a batch sharing the tets (strides in doubles / complex
+// values between consecutive meshes' nodes, materials and records; 0 for one mesh).
 template <bool REF_GEOM>
 __global__ void __launch_bounds__(256) k_tet_gram(int64_t nt, const double *__restrict__ nodes,
                                                   const int32_t *__restrict__ tets, const cplx *__restrict__ eps,
-                                                  const cplx *__restrict__ nu, double *__restrict__ rec) {
+                                                  const cplx *__restrict__ nu, double *__restrict__ rec,
+                                                  int64_t node_stride, int64_t mat_stride, int64_t rec_stride) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= nt) return;
+    nodes += blockIdx.y * node_stride;
+    eps += blockIdx.y * mat_stride;
+    nu += blockIdx.y * mat_stride;
+    rec += blockIdx.y * rec_stride;
     const int4 tv = reinterpret_cast<const int4 *>(tets)[t];
     int vid[4] = {tv.x, tv.y, tv.z, tv.w};
     V3 P[4], gr[4];
@@ -1083,7 +1093,11 @@ __global__ void __launch_bounds__(256) k_tet_gram(int64_t nt, const double *__re
 template <int TS, int CH>
 __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
     k_assemble_staged(const emg_pattern pat, const double *__restrict__ rec, int exact_face_mass,
-                      cplx *__restrict__ E, cplx *__restrict__ B) {
+                      cplx *__restrict__ E, cplx *__restrict__ B, int64_t rec_stride, int64_t out_stride) {
+    // blockIdx.y: the mesh of a batch (its records and output values at these strides)
+    rec += blockIdx.y * rec_stride;
+    E += blockIdx.y * out_stride;
+    B += blockIdx.y * out_stride;
     constexpr int TSP = TS + 1;             // odd field stride of the Gram data: fewer LDS bank conflicts
     constexpr int TD = TD_FIELDS * TSP;     // doubles of Gram data
     constexpr int SD = 2 * 4 * CH;          // doubles of one staged matrix: 4 blocks x CH complex
@@ -1245,17 +1259,17 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
 }
 
 template <int CH>
-static void launch_staged(const emg_pattern *pat, unsigned grid, hipStream_t st, const double *rec, int efm,
-                          cplx *oE, cplx *oB) {
+static void launch_staged(const emg_pattern *pat, dim3 grid, hipStream_t st, const double *rec, int efm, cplx *oE,
+                          cplx *oB, int64_t rec_stride = 0, int64_t out_stride = 0) {
     const int tm = pat->chunk_tmax;
     if (tm <= 32)
-        k_assemble_staged<32, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB);
+        k_assemble_staged<32, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB, rec_stride, out_stride);
     else if (tm <= 48)
-        k_assemble_staged<48, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB);
+        k_assemble_staged<48, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB, rec_stride, out_stride);
     else if (tm <= 64)
-        k_assemble_staged<64, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB);
+        k_assemble_staged<64, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB, rec_stride, out_stride);
     else
-        k_assemble_staged<CHUNK_TMAX, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB);
+        k_assemble_staged<CHUNK_TMAX, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB, rec_stride, out_stride);
 }
 
 static int g_num_cus = 0;
@@ -1293,10 +1307,12 @@ extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, c
         cplx *oE = reinterpret_cast<cplx *>(d_E), *oB = reinterpret_cast<cplx *>(d_B);
         if (!mp->tet_rec) EMG_HIP(hipMalloc(&mp->tet_rec, (size_t)pat->nt * REC * sizeof(double)));
         if (flags & EMG_REFERENCE_GEOMETRY)
-            k_tet_gram<true><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec);
+            k_tet_gram<true><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec,
+                                                                         0, 0, 0);
         else
-            k_tet_gram<false><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec);
-        launch_staged<CHUNK>(pat, (unsigned)grid, st, mp->tet_rec, efm, oE, oB);
+            k_tet_gram<false><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec,
+                                                                          0, 0, 0);
+        launch_staged<CHUNK>(pat, dim3((unsigned)grid), st, mp->tet_rec, efm, oE, oB);
         EMG_LAUNCH_CHECK();
         return 0;
     }
@@ -1310,6 +1326,50 @@ extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, c
                                                                    reinterpret_cast<const cplx *>(d_nu),
                                                                    reinterpret_cast<cplx *>(d_E),
                                                                    reinterpret_cast<cplx *>(d_B));
+    EMG_LAUNCH_CHECK();
+    return 0;
+}
+
+// nbatch meshes with the pattern's tets and their own node coordinates and materials, in
+// one pair of launches: mesh b's nodes at d_nodes + b * 3 nv (nv x 3 each), eps and nu at
+// + b * nt, E and B values at + b * nnz.  The batch's records live in a buffer of the
+// pattern that grows to the largest batch seen.
+extern "C" int emg_assemble_iso_batched(const emg_pattern *pat, int nbatch, int64_t nv, const double *d_nodes,
+                                        const int32_t *d_tets, const double *d_eps, const double *d_nu, int flags,
+                                        double *d_E, double *d_B, void *stream) {
+    EMG_CHECK(pat && d_nodes && d_tets && d_eps && d_nu && d_E && d_B, "emg_assemble_iso_batched: null argument");
+    EMG_CHECK(nbatch >= 1 && nbatch <= 65535, "emg_assemble_iso_batched: nbatch must be in [1, 65535]");
+    EMG_CHECK(nv > 0, "emg_assemble_iso_batched: nv must be positive");
+    EMG_CHECK(pat->fused_ok, "emg_assemble_iso_batched: the pattern has no fused-path chunk tables");
+    EMG_CHECK(!(flags & ~(EMG_EXACT_FACE_MASS | EMG_REFERENCE_GEOMETRY)), "emg_assemble_iso_batched: unsupported flag");
+    hipStream_t st = (hipStream_t)stream;
+    emg_pattern *mp = const_cast<emg_pattern *>(pat);
+    if (g_num_cus == 0) {
+        int dev = 0;
+        EMG_HIP(hipGetDevice(&dev));
+        EMG_HIP(hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    if (mp->rec_batch_cap < nbatch) {
+        if (mp->tet_rec_batch) EMG_HIP(hipFree(mp->tet_rec_batch));
+        mp->tet_rec_batch = nullptr;
+        mp->rec_batch_cap = 0;
+        EMG_HIP(hipMalloc(&mp->tet_rec_batch, (size_t)nbatch * pat->nt * REC * sizeof(double)));
+        mp->rec_batch_cap = nbatch;
+    }
+    // every mesh gets the single-mesh grid (at most 32 workgroups per CU): the batch is the
+    // single launches back to back without their gaps, and small meshes, whose own launch
+    // cannot fill the GPU, fill it together
+    const int64_t gx = std::max<int64_t>(1, std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * 32));
+    const int efm = (flags & EMG_EXACT_FACE_MASS) ? 1 : 0;
+    const cplx *ce = reinterpret_cast<const cplx *>(d_eps), *cn = reinterpret_cast<const cplx *>(d_nu);
+    cplx *oE = reinterpret_cast<cplx *>(d_E), *oB = reinterpret_cast<cplx *>(d_B);
+    const int64_t nvs = 3 * nv, rs = pat->nt * (int64_t)REC, os = 4 * pat->nnz_ent;
+    const dim3 gg(emg::grid_for(pat->nt, 256), (unsigned)nbatch);
+    if (flags & EMG_REFERENCE_GEOMETRY)
+        k_tet_gram<true><<<gg, 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec_batch, nvs, pat->nt, rs);
+    else
+        k_tet_gram<false><<<gg, 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec_batch, nvs, pat->nt, rs);
+    launch_staged<CHUNK>(pat, dim3((unsigned)gx, (unsigned)nbatch), st, mp->tet_rec_batch, efm, oE, oB, rs, os);
     EMG_LAUNCH_CHECK();
     return 0;
 }

@@ -75,6 +75,16 @@ int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, const int32_
                      const double *d_eps, const double *d_nu, int flags,
                      double *d_E, double *d_B, void *stream);
 
+/* nbatch meshes that share the pattern's tets (same topology, own node coordinates and
+ * materials: geometry variants, parameter studies) assembled in one pair of fused launches:
+ * mesh b's nodes at d_nodes + 3 nv b (nv x 3 float64 each), eps and nu at + nt b (complex128),
+ * E and B values at + nnz b.  Each mesh's values equal those of emg_assemble_iso on it.  No
+ * reference counterpart (the reference assembles one mesh per call); flags:
+ * EMG_EXACT_FACE_MASS, EMG_REFERENCE_GEOMETRY.                                            */
+int emg_assemble_iso_batched(const emg_pattern *pat, int nbatch, int64_t nv, const double *d_nodes,
+                             const int32_t *d_tets, const double *d_eps, const double *d_nu, int flags, double *d_E,
+                             double *d_B, void *stream);
+
 /* Same with full 3x3 complex tensors per tet: d_mm (nt,3,3) c128 multiplies
  * the mass term (eps_r), d_ms (nt,3,3) c128 the curl term (the reference
  * passes matinv(mu_r), fem/physics/edm/optimized_assembly.py:103-107).       */
