@@ -80,7 +80,27 @@ def sweep_workload(name: str):
         return mesh, lambda: sweep_ports(mesh), (2e9, 4e9), (
             "posts box 40x20x10 mm, 4 PEC posts r 1.5 mm h 8 mm, 80x40x16 Kuhn hexes "
             f"carved to {mesh.n_tets:,} tets, 2 lumped ports 2x3.75 mm (50 ohm) on the x faces, air")
+    if name == "patch":
+        from emerge_amd import workloads
+        mesh = meshgen.patch_antenna(seed=4)
+        bcs = lambda: workloads.patch_boundary_conditions()
+        bcs.materials = workloads.patch_materials()
+        return mesh, bcs, (2e9, 3e9), (
+            "microstrip patch 38x29 mm with inset feed on a 60x60 mm ground, eps_r 4.4 tan d 0.02 1.6 mm "
+            f"substrate, 90x90x39 mm air box with a first-order absorbing boundary, {mesh.n_tets:,} tets (seed 4), "
+            "one 50-ohm lumped port")
     raise ValueError(name)
+
+
+def make_phys(mesh, ports):
+    """Electrodynamics3D of a sweep workload: its boundary conditions and, where the
+    workload has them, its materials."""
+    from emerge_amd.physics import Electrodynamics3D
+    phys = Electrodynamics3D(mesh)
+    for tag, mat in getattr(ports, "materials", {}).items():
+        phys.set_material(tag, mat)
+    phys.assign(*ports())
+    return phys
 
 
 def sweep_ports(mesh):
@@ -127,8 +147,7 @@ def bench_sweep(args, dev):
     for key, mode, prec, fl in (("basis_mixed", "basis", 32, freqs), ("plain_fp64", "plain", 64, plain_f)):
         if not fl or mode not in runs_wanted:
             continue
-        phys = Electrodynamics3D(mesh)
-        phys.assign(*ports())
+        phys = make_phys(mesh, ports)
         phys.set_frequency(fl)
         st = phys.solveroutine
         st.precision, st.tol, st.store_fields, st.sweep, st.verbose = prec, args.sweep_tol, False, mode, True
@@ -157,8 +176,7 @@ def bench_sweep(args, dev):
     if args.learned and "learned" in runs_wanted:
         from emerge_amd.fdsolve import GPUSolver
         from emerge_amd.learned import LearnedSmoother
-        phys = Electrodynamics3D(mesh)
-        phys.assign(*ports())
+        phys = make_phys(mesh, ports)
         prob = phys.problem()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -230,8 +248,7 @@ def residual_kernel_roofline(mesh, ports, freqs, tol, nvec: int = 16, reps: int 
     from emerge_amd.physics import Electrodynamics3D
     from emerge_amd.sparam import PortSampler
     from emerge_amd.sweep import BasisSweep
-    phys = Electrodynamics3D(mesh)
-    phys.assign(*ports())
+    phys = make_phys(mesh, ports)
     prob = phys.problem()
     bs = BasisSweep(prob, GPUSolver(prob, precision=64, preconditioner="jacobi"), PortSampler(prob), max_basis=8)
     rng = np.random.default_rng(0)
@@ -349,8 +366,7 @@ def spmv_kernel_roofline(mesh, ports, freq, reps: int = 50):
     from emerge_amd import _lib
     from emerge_amd.fdsolve import GPUSolver
     from emerge_amd.physics import Electrodynamics3D
-    phys = Electrodynamics3D(mesh)
-    phys.assign(*ports())
+    phys = make_phys(mesh, ports)
     prob = phys.problem()
     solver = GPUSolver(prob, precision=32, preconditioner="jacobi", poly_degree=1)
     solver.set_matrix(prob.system(freq))
@@ -518,7 +534,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-face-mass", action="store_true")
     ap.add_argument("--two-phase", action="store_true", help="time the per-tet-block fallback kernels")
-    ap.add_argument("--sweep", default="posts", choices=["none", "wr90", "posts"],
+    ap.add_argument("--sweep", default="posts", choices=["none", "wr90", "posts", "patch"],
                     help="frequency-sweep workload reported beside the assembly figure")
     ap.add_argument("--sweep-freqs", type=int, default=201)
     ap.add_argument("--plain-sample", type=int, default=3,
@@ -556,8 +572,7 @@ def main():
     if args.cpu_sweep_child:
         from emerge_amd.physics import Electrodynamics3D
         mesh, ports, _, desc = sweep_workload(args.cpu_sweep_child)
-        phys = Electrodynamics3D(mesh)
-        phys.assign(*ports())
+        phys = make_phys(mesh, ports)
         print(json.dumps(cpu_sweep_baseline(args.cpu_sweep_child, mesh, phys, args.cpu_sweep_freq, desc)), flush=True)
         return
     if args.gpus != 1:

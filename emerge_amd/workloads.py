@@ -76,18 +76,27 @@ def cavity(nx=48, ny=32, nz=24, seed=3):
 PATCH_FREQS = tuple(np.linspace(2e9, 3e9, 101))
 
 
-def patch(seed=4, freqs=PATCH_FREQS, er_sub=4.4, tand=0.02, scale_w=1.0, scale_l=1.0, **mesh_kw):
+def patch_boundary_conditions(h_sub=1.6e-3, port_y=-27.5e-3, line_w=2.5e-3):
+    """The patch antenna's 50-ohm lumped port across the substrate at the line's end and the
+    first-order absorbing boundary on the six faces of the air box (new objects per call)."""
     from .bc import AbsorbingBoundary
-    kw = dict(mesh_kw)
-    mesh = meshgen.patch_antenna(seed=seed, scale_w=scale_w, scale_l=scale_l, **kw)
-    h_sub = kw.get("h_sub", 1.6e-3)
-    port_y = kw.get("port_y", -27.5e-3)
-    line_w = kw.get("line_w", 2.5e-3)
-    phys = Electrodynamics3D(mesh)
-    phys.set_material(meshgen.SUBSTRATE_TAG, Material(er=er_sub, tand=tand))
     cs = CoordinateSystem((1, 0, 0), (0, 0, 1), (0, -1, 0), (0.0, port_y, -h_sub / 2))
     port = LumpedPort([meshgen.FEED_PORT_TAG], 1, width=line_w, height=h_sub, direction=(0, 0, 1), cs=cs, Z0=50)
     abc = AbsorbingBoundary([meshgen.XMIN, meshgen.XMAX, meshgen.YMIN, meshgen.YMAX, meshgen.ZMIN, meshgen.ZMAX])
-    phys.assign(port, abc)
+    return [port, abc]
+
+
+def patch_materials(er_sub=4.4, tand=0.02) -> dict:
+    return {meshgen.SUBSTRATE_TAG: Material(er=er_sub, tand=tand)}
+
+
+def patch(seed=4, freqs=PATCH_FREQS, er_sub=4.4, tand=0.02, scale_w=1.0, scale_l=1.0, **mesh_kw):
+    kw = dict(mesh_kw)
+    mesh = meshgen.patch_antenna(seed=seed, scale_w=scale_w, scale_l=scale_l, **kw)
+    phys = Electrodynamics3D(mesh)
+    for tag, mat in patch_materials(er_sub, tand).items():
+        phys.set_material(tag, mat)
+    phys.assign(*patch_boundary_conditions(kw.get("h_sub", 1.6e-3), kw.get("port_y", -27.5e-3),
+                                           kw.get("line_w", 2.5e-3)))
     phys.set_frequency(list(freqs))
     return phys
