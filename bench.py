@@ -434,7 +434,7 @@ def learned_bound(name: str, log) -> dict:
             for d in degrees:
                 for wname, w in (("plain", (None, None)), ("optimal", (we, wp))):
                     s = GPUSolver(prob, precision=32, poly_degree=d)
-                    s.POLY_STEP_CAP = maxit
+                    s.POLY_STEP_CAP = s.POLY_CHUNK = maxit
                     its, conv, res = [], [], []
                     for port in prob.ports:
                         s.set_weights(*w)

@@ -275,7 +275,9 @@ class GPUSolver:
             self.h = None
 
     learned = None          # LearnedSmoother whose weights scale the smoother at each frequency
-    POLY_STEP_CAP = 3000    # GMRES steps allowed with the polynomial before the plain smoother takes over
+    POLY_STEP_CAP = 3000    # steps at the configured degree before it is halved for good
+    POLY_CHUNK = 1000       # GMRES steps between progress checks while the polynomial is active
+    POLY_MIN_GAIN = 2.0     # a chunk must cut the residual this much, else the degree halves
 
     def spmv_tuning(self) -> dict:
         """The autotuned SpMV shape of this solver (chosen on its first matrix): lanes per
@@ -337,25 +339,52 @@ class GPUSolver:
         self.poly_active = d
 
     def solve(self, b, x0=None, tol: float = 1e-8, maxiter: int = 20000, max_refine: int = 0):
-        """GMRES from x0.  With the polynomial active, a solve that has not converged after
-        POLY_STEP_CAP steps continues from its last iterate without it (info.reserved bit 2;
-        info.iterations then counts smoother applications).  With learned smoother weights
-        active, a solve that does not converge is continued from its last iterate with the
-        plain smoother (bit 1)."""
+        """GMRES from x0.  maxiter bounds the smoother applications (a step with the degree-d
+        polynomial costs d).  While the polynomial is active the solve runs in chunks of
+        POLY_CHUNK steps, each continuing from the last iterate.  The degree halves (6 -> 3
+        -> 1) for this and every later solve of this solver when the configured degree has
+        not converged in POLY_STEP_CAP steps, or when a chunk at any degree does not cut the
+        FP64 residual POLY_MIN_GAIN-fold (complex64 steps can stall with a high-degree
+        polynomial on hard matrices); info.reserved bit 2 is set and info.iterations then
+        counts smoother applications.  With learned smoother weights active, a solve that
+        does not converge is continued from its last iterate with the plain smoother (bit 1)."""
         torch = self.p.torch
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         info = _lib.SolveInfo()
-        cap = maxiter if self.poly_active <= 1 else min(maxiter, self.POLY_STEP_CAP)
-        _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, cap, max_refine,
-                                           ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
-        if self.poly_active > 1 and not info.converged:
-            # the polynomial did not pay off for this matrix: continue with the plain smoother
-            used = info.iterations * self.poly_active
-            _lib.check(self.L.emg_solver_set_poly(self.h, 1, None), "emg_solver_set_poly")
-            self.poly_active = 1
-            _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, maxiter, max_refine,
+        apps = steps = steps_d = 0
+        lowered = False
+        res_prev = None
+        while True:
+            d = max(1, self.poly_active)
+            budget = (maxiter - apps) // d
+            if budget <= 0:
+                break
+            n = min(budget, self.POLY_CHUNK) if d > 1 else budget
+            _lib.check(self.L.emg_solver_solve(self.h, _lib.ptr(b), _lib.ptr(x), tol, n, max_refine,
                                                ctypes.byref(info), _lib.stream_ptr()), "emg_solver_solve")
-            info.iterations += used
+            apps += info.iterations * d
+            steps += info.iterations
+            steps_d += info.iterations
+            if info.converged or d == 1:
+                break
+            stalled = res_prev is not None and info.residual * self.POLY_MIN_GAIN > res_prev
+            capped = not getattr(self, "_poly_lowered", False) and steps_d >= self.POLY_STEP_CAP
+            if stalled or capped:
+                # the polynomial does not pay off on this matrix: halve its degree from here on
+                self._poly_lowered = True
+                steps_d = 0
+                self.poly_degree = d // 2
+                if self.poly_degree > 1:
+                    self._update_poly()
+                else:
+                    _lib.check(self.L.emg_solver_set_poly(self.h, 1, None), "emg_solver_set_poly")
+                    self.poly_active = 1
+                lowered = True
+                res_prev = None
+                continue
+            res_prev = info.residual
+        info.iterations = apps if lowered else steps
+        if lowered:
             info.reserved |= 2
         w = getattr(self, "_w", (None, None))
         if not info.converged and (w[0] is not None or w[1] is not None):

@@ -318,7 +318,9 @@ class BasisSweep:
                                           max_refine=self.max_refine)
             self.solver.learned = learned
             self.solver.set_matrix(p.K, freq)
-            cap = max(1, int(info2.iterations) - 1)
+            # the cap in smoother applications (what maxiter bounds): steps times the degree used
+            apps2 = int(info2.iterations) if info2.reserved & 2 else int(info2.iterations) * max(1, self.solver.poly_active)
+            cap = max(1, apps2 - 1)
             x, info = self.solver.solve(b, x0=x0, tol=self.tol * self.basis_tol, maxiter=cap,
                                         max_refine=self.max_refine)
             use = bool(info.converged) and not (info.reserved & 1) and info.iterations < info2.iterations
@@ -337,7 +339,8 @@ class BasisSweep:
             import sys
             print(f"sweep: GMRES f={freq:.6g} port {self.ports[port_index].port_number}: {info.iterations} iterations, "
                   f"residual {info.residual:.2e}{' (plain smoother fallback)' if info.reserved & 1 else ''}"
-                  f"{' (polynomial dropped)' if info.reserved & 2 else ''}", file=sys.stderr,
+                  f"{f' (polynomial degree lowered to {self.solver.poly_degree})' if info.reserved & 2 else ''}",
+                  file=sys.stderr,
                   flush=True)
         return x, info
 
