@@ -101,6 +101,18 @@ class Pardiso:
         self._call(-1, np.zeros(1), np.zeros(1), 1)
 
 
+def inertia(S: sp.spmatrix, threads: int | None = None) -> tuple[int, int]:
+    """(positive, negative) eigenvalue counts of a real symmetric matrix, from PARDISO's
+    symmetric indefinite factorisation (mtype -2, upper triangle; iparm(22), iparm(23)).
+    With S = A - tau B (B positive definite) the negative count is the number of
+    generalized eigenvalues below tau (Sylvester's law of inertia)."""
+    U = sp.triu(sp.csr_matrix(S)).tocsr()
+    P = Pardiso(U, threads=threads, mtype=-2)
+    out = int(P.iparm[21]), int(P.iparm[22])
+    P.release()
+    return out
+
+
 def solve_complex(Asel: sp.csr_matrix, bsel: np.ndarray, threads: int | None = None) -> np.ndarray:
     """SolverPardiso.solve on the real block system (fem/solver.py:304-309, 437-464)."""
     Ab, bb = complex_to_real_block(Asel, bsel)

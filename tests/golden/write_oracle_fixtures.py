@@ -1,7 +1,7 @@
 """Write the oracle's answers for the configured workloads at full size, for the
 GPU parity tests that cannot run the oracle inside their time budget.
 
-    python tests/golden/write_oracle_fixtures.py c1 c3 c4
+    python tests/golden/write_oracle_fixtures.py c1 c3 c4 c4_inertia
 
 * c1: WR-90 section (20,160 tets) at 10 GHz: S (2 x 2) from the oracle's
       SuperLU direct solve with the reference's settings (fem/solver.py:243-275).
@@ -10,8 +10,18 @@ GPU parity tests that cannot run the oracle inside their time budget.
       (indices 0, 20, ..., 200 of 201 points over 2-4 GHz) from MKL PARDISO on
       the real block system (fem/solver.py:70-90, 294-309).
 * c4: cavity with the eps_r = 6 block (221,184 tets): the 10 eigenvalues
-      k^2 nearest (2 pi 5 GHz / c)^2 from ARPACK shift-invert with a PARDISO
+      k^2 nearest (2 pi 5 GHz / c)^2, from ARPACK shift-invert with a PARDISO
       factorisation (fem/physics/edm/emfreq3d.py:838-848), gradient modes removed.
+      Every mode of this cavity lies above 5 GHz, so these are its 10 lowest modes;
+      at the 5 GHz shift itself the gradient null space (lambda = 0, |nu| = 1/sigma)
+      outranks them all in ARPACK's ordering and the iteration does not settle.  The
+      modes come from two shifts, 9.5 and 13 GHz, 6 modes each: every mode found lies
+      within half the null space's distance |lambda - sigma| < sigma / 2 of its shift,
+      the two sets overlap (no mode between them can be missed), and the 9.5 GHz set
+      holds the lowest modes.
+* c4_inertia: completeness of those 10 by Sylvester's law of inertia (PARDISO's
+      symmetric indefinite factorisation of A - tau B): no physical mode below 5 GHz,
+      exactly 10 up to the 10th.
 
 Results go to tests/golden/oracle_fixtures.npz (keys are replaced one
 workload at a time).  Every number comes from the CPU oracle in oracle/.
@@ -75,15 +85,65 @@ def c4():
     phys = workloads.cavity()
     er, ur = phys.material_tensors()
     orc = fd.OracleProblem(phys.mesh, er, ur, phys.boundary_conditions)
-    sigma = (2 * np.pi * workloads.CAVITY_TARGET_HZ / C0) ** 2
-    t0 = time.perf_counter()
     print(f"c4: {phys.mesh.n_tets} tets", flush=True)
-    w = fd.oracle_eigenmodes(orc, sigma, 10, solver="pardiso", threads=len(os.sched_getaffinity(0)), extra=6,
-                             verbose=True)
-    print(f"c4 eigenvalues {np.sort(w.real)} ({time.perf_counter() - t0:.1f} s)", flush=True)
-    _save(c4_sigma=np.array(sigma), c4_lambda=w)
+    sigma = (2 * np.pi * workloads.CAVITY_TARGET_HZ / C0) ** 2
+    threads = len(os.sched_getaffinity(0))
+    runs = []
+    for f_shift in (9.5e9, 13e9):
+        s = (2 * np.pi * f_shift / C0) ** 2
+        t0 = time.perf_counter()
+        w = np.sort(fd.oracle_eigenmodes(orc, s, 6, solver="pardiso", threads=threads, extra=0, verbose=True).real)
+        print(f"c4 shift {f_shift / 1e9:g} GHz: {np.sqrt(w) * C0 / (2 * np.pi) / 1e9} GHz "
+              f"({time.perf_counter() - t0:.1f} s)", flush=True)
+        assert np.all(np.abs(w - s) < 0.5 * s), (f_shift, w)
+        runs.append(w)
+    lo, hi = runs
+    assert hi.min() <= lo.max() * (1 + 1e-10), (lo, hi)                 # the two sets overlap
+    both = np.sort(np.concatenate([lo, hi]))
+    keep = np.concatenate([[True], np.diff(both) > 1e-9 * both[1:]])     # a mode found twice counts once
+    lam = both[keep][:10]
+    assert lam.shape[0] == 10 and lam.max() <= hi.max(), lam
+    _save(c4_sigma=np.array(sigma), c4_lambda=lam.astype(np.complex128), c4_shifts_hz=np.array([9.5e9, 13e9]))
+
+
+def c4_inertia():
+    """Completeness of the c4 eigenvalues by Sylvester's law of inertia: the negative
+    eigenvalues of A - tau B (PEC-reduced, symmetric, B positive definite) are the
+    generalized eigenvalues below tau, the gradient null space (lambda = 0) among them;
+    its dimension is the number of interior vertices plus interior edges (the P2 Lagrange
+    space vanishing on the boundary, whose gradients span the kernel of the curl in this
+    simply connected cavity).  tau at 5 GHz must leave no physical mode below, and tau
+    between the 10th and 11th eigenvalue exactly 10."""
+    from emerge_amd import workloads
+    from emerge_amd.bc import C0
+    from oracle import fd, pardiso
+    g = np.load(OUT)
+    lam = np.sort(g["c4_lambda"].real)
+    phys = workloads.cavity()
+    er, ur = phys.material_tensors()
+    orc = fd.OracleProblem(phys.mesh, er, ur, phys.boundary_conditions)
+    _, _, solve_ids, _ = orc.assemble_freq_matrix(1.0)
+    A = orc.E[solve_ids][:, solve_ids].real.tocsr()
+    B = orc.B[solve_ids][:, solve_ids].real.tocsr()
+    m = phys.mesh
+    bnd = m.tri_to_tet[1] < 0
+    bnd_nodes = np.unique(m.tris[:, bnd])
+    bnd_edges = np.unique(m.tri_to_edge[:, bnd])
+    n_grad = (m.n_nodes - bnd_nodes.shape[0]) + (m.n_edges - bnd_edges.shape[0])
+    threads = len(os.sched_getaffinity(0))
+    sigma = (2 * np.pi * workloads.CAVITY_TARGET_HZ / C0) ** 2
+    counts = []
+    for tau in (sigma, lam[9] * (1 + 1e-6)):
+        t0 = time.perf_counter()
+        npos, nneg = pardiso.inertia(A - tau * B, threads=threads)
+        counts.append(nneg - n_grad)
+        print(f"c4 inertia: tau {np.sqrt(tau) * C0 / (2 * np.pi) / 1e9:.4f} GHz: {nneg} negative, "
+              f"{npos} positive of {A.shape[0]}; gradient space {n_grad}; physical modes below: {nneg - n_grad} "
+              f"({time.perf_counter() - t0:.1f} s)", flush=True)
+    assert counts[0] == 0 and counts[1] == 10, counts
+    _save(c4_modes_below=np.array(counts))
 
 
 if __name__ == "__main__":
     for name in sys.argv[1:]:
-        {"c1": c1, "c3": c3, "c4": c4}[name]()
+        {"c1": c1, "c3": c3, "c4": c4, "c4_inertia": c4_inertia}[name]()

@@ -205,3 +205,25 @@ def test_oracle_curl_interpolation_by_finite_differences():
     fdcurl = np.array([J[2, 1] - J[1, 2], J[0, 2] - J[2, 0], J[1, 0] - J[0, 1]])
     an = prob.interpolate_curl(x, p0[:, None], [t])[:, 0]
     assert np.abs(an - fdcurl).max() <= 1e-6 * np.abs(an).max()
+
+
+def test_pardiso_inertia_counts_eigenvalues_below_the_shift():
+    """oracle.pardiso.inertia (the completeness check of the cavity fixture) against dense
+    generalized eigenvalues: the negative count of A - tau B is the number below tau."""
+    import scipy.linalg as sla
+    import scipy.sparse as sp
+    from oracle import pardiso
+    if not pardiso.available():
+        pytest.skip("MKL PARDISO is not available")
+    rng = np.random.default_rng(5)
+    n = 60
+    X = rng.standard_normal((n, n))
+    A = X @ X.T
+    A[:, :8] = 0.0
+    A[:8, :] = 0.0                                  # an 8-dimensional null space, as the curl-curl has
+    Y = rng.standard_normal((n, n))
+    B = Y @ Y.T + n * np.eye(n)
+    lam = sla.eigh(A, B, eigvals_only=True)
+    for tau in (0.5 * lam[8] if lam[8] > 0 else 1e-3, 0.5 * (lam[20] + lam[21]), lam[-1] * 1.01):
+        npos, nneg = pardiso.inertia(sp.csr_matrix(A - tau * B), threads=1)
+        assert nneg == int(np.sum(lam < tau)) and npos + nneg == n, (tau, npos, nneg)
