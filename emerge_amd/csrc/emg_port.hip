@@ -318,11 +318,12 @@ struct Blas {
 };
 Blas g_blas;
 
-// one rocBLAS handle per device, bound to the caller's stream on every use
+// one rocBLAS handle per device, bound to the caller's stream on every use; the caller
+// holds g_blas.mu for as long as it uses the handle (entry points on one device from
+// several host threads run one at a time)
 rocblas_handle blas_for(hipStream_t stream, int *err) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) { *err = 1; return nullptr; }
-    std::lock_guard<std::mutex> lk(g_blas.mu);
     if ((int)g_blas.h.size() <= dev) g_blas.h.resize(dev + 1, nullptr);
     if (!g_blas.h[dev] && rocblas_create_handle(&g_blas.h[dev]) != rocblas_status_success) { *err = 2; return nullptr; }
     if (rocblas_set_stream(g_blas.h[dev], stream) != rocblas_status_success) { *err = 3; return nullptr; }
@@ -388,6 +389,7 @@ extern "C" int emg_port_factor(int64_t n, const double *d_A, const double *d_B, 
     EMG_CHECK(n > 0 && n < (1 << 30) && n * n < (1ll << 40), "emg_port_factor: bad size");
     hipStream_t s = (hipStream_t)stream;
     int err = 0;
+    std::lock_guard<std::mutex> blas_lock(g_blas.mu);
     rocblas_handle h = blas_for(s, &err);
     EMG_CHECK(h, "emg_port_factor: no rocBLAS handle");
     k_shift<<<emg::grid_for(n * n, 256), 256, 0, s>>>(n * n, reinterpret_cast<const cplx *>(d_A),
@@ -415,6 +417,7 @@ extern "C" int emg_port_krylov(int64_t n, const double *d_M, const int *d_ipiv, 
     EMG_CHECK(n > 0 && n < (1 << 30) && m >= 1 && m < n + 1 && m < 4096, "emg_port_krylov: bad sizes");
     hipStream_t s = (hipStream_t)stream;
     int err = 0;
+    std::lock_guard<std::mutex> blas_lock(g_blas.mu);
     rocblas_handle h = blas_for(s, &err);
     EMG_CHECK(h, "emg_port_krylov: no rocBLAS handle");
     using Z = rocblas_double_complex;
@@ -479,6 +482,7 @@ extern "C" int emg_port_inverse_step(int64_t n, const double *d_M, const int *d_
     EMG_CHECK(n > 0 && n < (1 << 30) && k >= 0 && k < (1 << 30), "emg_port_inverse_step: bad sizes");
     if (k == 0) return 0;
     int err = 0;
+    std::lock_guard<std::mutex> blas_lock(g_blas.mu);
     rocblas_handle h = blas_for((hipStream_t)stream, &err);
     EMG_CHECK(h, "emg_port_inverse_step: no rocBLAS handle");
     using Z = rocblas_double_complex;
@@ -502,6 +506,7 @@ extern "C" int emg_zgemm(int transa, int transb, int64_t m, int64_t n, int64_t k
     EMG_CHECK(lda >= 1 && ldb >= 1 && ldc >= (m > 1 ? m : 1), "emg_zgemm: bad leading dimension");
     if (m == 0 || n == 0) return 0;
     int err = 0;
+    std::lock_guard<std::mutex> blas_lock(g_blas.mu);
     rocblas_handle h = blas_for((hipStream_t)stream, &err);
     EMG_CHECK(h, "emg_zgemm: no rocBLAS handle");
     using Z = rocblas_double_complex;
