@@ -9,7 +9,7 @@
  *
  * Each function cites the reference lines it restates.  The formulas are
  * kept term for term (same index choices, same block mirroring, same matinv
- * quirk) because parity is judged against exactly these expressions.
+ * quirk) because parity is measured against exactly these expressions.
  *
  * Built by oracle/Makefile into oracle/liboracle.so (gcc -O2 -fopenmp).
  */
