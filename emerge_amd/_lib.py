@@ -55,7 +55,8 @@ SIGNATURES: dict[str, tuple] = {
     "emg_port_field": (ctypes.c_int, [I64, P, I64, P, I64, P, P, I64, P, F64, P, P, F64, F64, P, P, P]),
     "emg_port_factor": (ctypes.c_int, [I64, P, P, F64, F64, P, P, P]),
     "emg_port_inverse_step": (ctypes.c_int, [I64, P, P, P, I64, P, P, P]),
-    "emg_port_krylov": (ctypes.c_int, [I64, P, P, P, ctypes.c_int, P, P, P]),
+    "emg_port_operator": (ctypes.c_int, [I64, P, P, P, P, P]),
+    "emg_port_krylov": (ctypes.c_int, [I64, P, ctypes.c_int, P, P, P]),
     "emg_zgemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, I64, I64, I64, F64, F64, P, I64, P, I64, F64, F64, P, I64,
                                  P]),
     "emg_solver_set_weights": (ctypes.c_int, [P, P, P]),

@@ -15,9 +15,10 @@
 //    dropped.  The reference's quirks are kept: its 8-digit quadrature table,
 //    face rows of Att/Btt/Dtt copied from the face columns, and the face-face
 //    entries of Dtt left zero unless EMG_PORT_FACE_BLOCK asks for them.
-//  * emg_port_factor / emg_port_krylov: shift-invert Arnoldi on
-//    (A - sigma B)^{-1} B with a rocSOLVER LU (zgetrf / zgetrs) and rocBLAS
-//    matrix-vector products; classical Gram-Schmidt with one re-orthogonalisation.
+//  * emg_port_factor / emg_port_operator / emg_port_krylov: shift-invert Arnoldi on
+//    T = (A - sigma B)^{-1} B, formed once from a rocSOLVER LU (zgetrf, then one
+//    zgetrs with n right-hand sides), each step one rocBLAS zgemv; classical
+//    Gram-Schmidt with one re-orthogonalisation.
 //  * k_port_field: the face field (and the reference's curl expression) at
 //    points, each point taking the last triangle that contains it, as
 //    ned2_tri_interp_full / _curl do (fem/mth/tri.py:228-401).
@@ -411,18 +412,33 @@ extern "C" int emg_port_factor(int64_t n, const double *d_A, const double *d_B, 
     return 0;
 }
 
-extern "C" int emg_port_krylov(int64_t n, const double *d_M, const int *d_ipiv, const double *d_B, int m, double *d_V,
-                               double *d_H, void *stream) {
-    EMG_CHECK(d_M && d_ipiv && d_B && d_V && d_H, "emg_port_krylov: null argument");
+extern "C" int emg_port_operator(int64_t n, const double *d_M, const int *d_ipiv, const double *d_B, double *d_T,
+                                 void *stream) {
+    EMG_CHECK(d_M && d_ipiv && d_B && d_T && d_T != d_B && d_T != d_M, "emg_port_operator: null or aliased argument");
+    EMG_CHECK(n > 0 && n < (1 << 30) && n * n < (1ll << 40), "emg_port_operator: bad size");
+    hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> blas_lock(g_blas.mu);
+    int err = 0;
+    rocblas_handle h = blas_for(s, &err);
+    EMG_CHECK(h, "emg_port_operator: no rocBLAS handle");
+    using Z = rocblas_double_complex;
+    EMG_HIP(hipMemcpyAsync(d_T, d_B, sizeof(Z) * n * n, hipMemcpyDeviceToDevice, s));
+    EMG_BLAS(rocsolver_zgetrs(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                              const_cast<Z *>(reinterpret_cast<const Z *>(d_M)), (rocblas_int)n, d_ipiv,
+                              reinterpret_cast<Z *>(d_T), (rocblas_int)n));
+    return 0;
+}
+
+extern "C" int emg_port_krylov(int64_t n, const double *d_T, int m, double *d_V, double *d_H, void *stream) {
+    EMG_CHECK(d_T && d_V && d_H, "emg_port_krylov: null argument");
     EMG_CHECK(n > 0 && n < (1 << 30) && m >= 1 && m < n + 1 && m < 4096, "emg_port_krylov: bad sizes");
     hipStream_t s = (hipStream_t)stream;
-    int err = 0;
     std::lock_guard<std::mutex> blas_lock(g_blas.mu);
+    int err = 0;
     rocblas_handle h = blas_for(s, &err);
     EMG_CHECK(h, "emg_port_krylov: no rocBLAS handle");
     using Z = rocblas_double_complex;
-    const Z *M = reinterpret_cast<const Z *>(d_M);
-    const Z *B = reinterpret_cast<const Z *>(d_B);
+    const Z *T = reinterpret_cast<const Z *>(d_T);
     Z *V = reinterpret_cast<Z *>(d_V);
     Z *H = reinterpret_cast<Z *>(d_H);
     const int rows = m + 1;
@@ -449,10 +465,9 @@ extern "C" int emg_port_krylov(int64_t n, const double *d_M, const int *d_ipiv, 
     k_scale_by<<<emg::grid_for(n, 256), 256, 0, s>>>(n, d_nrm, reinterpret_cast<cplx *>(V));
     for (int j = 0; j < m && rc == 0; j++) {
         Z *w = V + (int64_t)(j + 1) * n;
-        fail(rocblas_zgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one, B, (rocblas_int)n,
-                           V + (int64_t)j * n, 1, &zero, w, 1), "zgemv B v");
-        fail(rocsolver_zgetrs(h, rocblas_operation_none, (rocblas_int)n, 1, const_cast<Z *>(M), (rocblas_int)n,
-                              d_ipiv, w, (rocblas_int)n), "zgetrs");
+        // w = T v_j, T = (A - sigma B)^{-1} B formed once by emg_port_operator
+        fail(rocblas_zgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one, T, (rocblas_int)n,
+                           V + (int64_t)j * n, 1, &zero, w, 1), "zgemv T v");
         for (int pass = 0; pass < 2; pass++) {
             fail(rocblas_zgemv(h, rocblas_operation_conjugate_transpose, (rocblas_int)n, j + 1, &one, V, (rocblas_int)n,
                                w, 1, &zero, d_h, 1), "zgemv V^H w");

@@ -15,7 +15,9 @@ with Assembler.assemble_bma_matrices (fem/physics/edm/assembler.py:246-308):
   -8 t^2 <= lambda <= -1 of filter_real_modes (fem/solver.py:34-68, called with
   ermax = urmax = 2 and t the target propagation constant).  Here the window is
   searched by shift-invert Arnoldi around its centre (rocSOLVER LU of
-  A - sigma B, ``emg_port_krylov``), followed by an FP64 Rayleigh-Ritz of the
+  A - sigma B, the operator (A - sigma B)^{-1} B formed once by
+  ``emg_port_operator``, one matrix-vector product per step of
+  ``emg_port_krylov``), followed by an FP64 Rayleigh-Ritz of the
   pencil on the Krylov basis; the basis grows until every Ritz value in the
   window has a relative residual below 1e-6, and each kept pair is then
   refined by two steps of inverse iteration at its own Ritz value (a second
@@ -212,6 +214,10 @@ class ModalSolve:
         ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
         _lib.check(self.L.emg_port_factor(n, _lib.ptr(self.A), _lib.ptr(self.B), sigma, 0.0, _lib.ptr(M),
                                           _lib.ptr(ipiv), _lib.stream_ptr()), "emg_port_factor")
+        T = torch.empty_like(self.A)                                              # (A - sigma B)^{-1} B
+        _lib.check(self.L.emg_port_operator(n, _lib.ptr(M), _lib.ptr(ipiv), _lib.ptr(self.B), _lib.ptr(T),
+                                            _lib.stream_ptr()), "emg_port_operator")
+        del M
         rng = np.random.default_rng(seed)
         m = min(m0, n)
         while True:
@@ -219,8 +225,8 @@ class ModalSolve:
             V = torch.zeros((m + 1, n), dtype=torch.complex128, device="cuda")
             V[0] = torch.from_numpy(rng.standard_normal(n) + 1j * rng.standard_normal(n)).cuda()
             H = torch.zeros((m, m + 1), dtype=torch.complex128, device="cuda")
-            _lib.check(self.L.emg_port_krylov(n, _lib.ptr(M), _lib.ptr(ipiv), _lib.ptr(self.B), m, _lib.ptr(V),
-                                              _lib.ptr(H), _lib.stream_ptr()), "emg_port_krylov")
+            _lib.check(self.L.emg_port_krylov(n, _lib.ptr(T), m, _lib.ptr(V), _lib.ptr(H), _lib.stream_ptr()),
+                       "emg_port_krylov")
             sub = torch.diagonal(H[:, 1:]).abs().cpu().numpy()                    # H[j + 1, j]
             brk = np.flatnonzero(sub <= 1e-14 * max(sub.max(), 1e-300))
             k = int(brk[0]) + 1 if brk.size else m

@@ -313,12 +313,17 @@ int emg_port_factor(int64_t n, const double *d_A, const double *d_B, double sigm
 int emg_port_inverse_step(int64_t n, const double *d_M, const int *d_ipiv, const double *d_B, int64_t k,
                           const double *d_X, double *d_Y, void *stream);
 
-/* m steps of Arnoldi on (A - sigma B)^{-1} B from the LU of emg_port_factor: d_V (n x (m+1)
+/* d_T = (A - sigma B)^{-1} d_B (n x n complex128, column-major) from the LU of
+ * emg_port_factor: the shift-invert operator of the port pencil, formed once (one zgetrs with
+ * n right-hand sides) so that each Arnoldi step is one matrix-vector product.           */
+int emg_port_operator(int64_t n, const double *d_M, const int *d_ipiv, const double *d_B, double *d_T,
+                      void *stream);
+
+/* m steps of Arnoldi on d_T (emg_port_operator): d_V (n x (m+1)
  * complex128, column 0 the start vector, normalised here) receives the orthonormal basis,
  * d_H ((m+1) x m complex128, column-major) the Hessenberg matrix; a breakdown leaves zero
  * columns and a zero subdiagonal entry.  Classical Gram-Schmidt, two passes.             */
-int emg_port_krylov(int64_t n, const double *d_M, const int *d_ipiv, const double *d_B, int m, double *d_V,
-                    double *d_H, void *stream);
+int emg_port_krylov(int64_t n, const double *d_T, int m, double *d_V, double *d_H, void *stream);
 
 /* C = alpha op(A) op(B) + beta C, complex128 column-major, op 0 = N, 1 = T, 2 = C
  * (rocBLAS zgemm): the Rayleigh-Ritz projections and Ritz vectors of the port solve.   */
