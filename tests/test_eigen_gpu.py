@@ -81,7 +81,9 @@ def test_eigen_refuses_unmet_residual(cuda):
 
 def test_cavity_workload_modes_match_oracle_fixture(cuda):
     """Full cavity workload (221,184 tets, eps_r 6 block): the 10 modes nearest 5 GHz equal the
-    oracle's (ARPACK shift-invert with PARDISO, tests/golden/oracle_fixtures.npz) as a sorted set."""
+    oracle's (ARPACK shift-invert with PARDISO, tests/golden/oracle_fixtures.npz) as a sorted set;
+    the fixture's inertia counts (no mode below 5 GHz, exactly 10 up to the 10th) make that set
+    the complete one."""
     import os
     from emerge_amd import workloads
     gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_fixtures.npz"))
@@ -89,6 +91,8 @@ def test_cavity_workload_modes_match_oracle_fixture(cuda):
         pytest.skip("the oracle's cavity eigenvalues are not in oracle_fixtures.npz "
                     "(tests/golden/write_oracle_fixtures.py c4 writes them)")
     lam_o = np.sort(gold["c4_lambda"].real)
+    if "c4_modes_below" in gold.files:
+        assert list(gold["c4_modes_below"]) == [0, 10]
     phys = workloads.cavity()
     print(f"cavity: {phys.mesh.n_tets} tets", flush=True)
     data = phys.eigenmode(num_sols=10, target_frequency=workloads.CAVITY_TARGET_HZ)
