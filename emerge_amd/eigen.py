@@ -32,7 +32,7 @@ from .bc import C0
 class ShiftInvertArnoldi:
     def __init__(self, problem, sigma: float, ncv: int = 40, inner_tol: float = 1e-12, restart: int = 150,
                  inner_maxiter: int = 200000, seed: int = 0, preconditioner: str = "hybrid", verbose: bool = False,
-                 poly_degree: int = 1):
+                 poly_degree: int = 1, precision: int = 64):
         import torch
         from .fdsolve import GPUSolver
         self.torch = torch
@@ -47,7 +47,9 @@ class ShiftInvertArnoldi:
         self.Ks = torch.empty(problem.nnz, dtype=torch.complex128, device=dev)
         _lib.check(self.L.emg_affine_combine(problem.nnz, _lib.ptr(problem.E), _lib.ptr(problem.B), self.sigma,
                                              _lib.ptr(self.Ks), _lib.stream_ptr()), "emg_affine_combine")
-        self.solver = GPUSolver(problem, restart=restart, precision=64, preconditioner=preconditioner,
+        # inner solves: complex128 GMRES (precision 64) or complex64 cycles in FP64 refinement (32),
+        # both to the FP64 residual inner_tol
+        self.solver = GPUSolver(problem, restart=restart, precision=precision, preconditioner=preconditioner,
                                 poly_degree=poly_degree)
         self.solver.set_matrix(self.Ks)
         self.V = torch.zeros((ncv + 1, N), dtype=torch.complex128, device=dev)

@@ -251,14 +251,16 @@ class Electrodynamics3D:
 
     def eigenmode(self, mesh: Mesh3D | None = None, solver=None, num_sols: int = 6,
                   target_frequency: float | None = None, ncv: int | None = None, inner_tol: float = 1e-12,
-                  spurious_rel: float = 1e-3) -> EMSimData:
+                  spurious_rel: float = 1e-3, inner_precision: int = 32) -> EMSimData:
         """Modes of E x = k^2 B x nearest k(target)^2 with PEC walls (emfreq3d.py:815-858).
 
         Same leading arguments as the reference's (mesh, solver, num_sols); the
         target is the first frequency of set_frequency (emfreq3d.py:837) unless
         target_frequency is given.  mesh: a mesh to solve on instead of the
         physics' own; solver: a GPUSolveRoutine whose preconditioner and
-        polynomial settings the inner solves take (None: solveroutine's).
+        polynomial settings the inner solves take (None: solveroutine's);
+        inner_precision: 64 (complex128 inner GMRES) or 32 (complex64 cycles in FP64
+        refinement), both to the FP64 residual inner_tol.
         Returns an EMSimData with one dataset per mode: freq (Hz), k0, mode,
         residual, and the mode field as _mode_field; raises EigenError if fewer
         than num_sols modes reach the residual target."""
@@ -276,7 +278,7 @@ class Electrodynamics3D:
         prob = self.problem()
         arn = ShiftInvertArnoldi(prob, sigma, ncv=ncv or max(4 * num_sols + 20, 24), inner_tol=inner_tol,
                                  preconditioner=st.preconditioner, verbose=getattr(st, "verbose", False),
-                                 poly_degree=st.poly_degree)
+                                 poly_degree=st.poly_degree, precision=inner_precision)
         lam, X, resid = arn.run(num_sols, spurious_rel=spurious_rel)
         data = EMSimData(basis=self.locator())
         for i in range(lam.shape[0]):

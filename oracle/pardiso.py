@@ -53,7 +53,7 @@ def complex_to_real_block(A, b):
 class Pardiso:
     """Real nonsymmetric PARDISO factorisation of a CSR matrix (zero-based indices)."""
 
-    def __init__(self, A: sp.csr_matrix, threads: int | None = None, mtype: int = 11):
+    def __init__(self, A: sp.csr_matrix, threads: int | None = None, mtype: int = 11, iparm: dict | None = None):
         L = mkl()
         if threads:
             L.MKL_Set_Num_Threads(ctypes.c_int(threads))
@@ -70,6 +70,8 @@ class Pardiso:
         self.iparm[0] = 1          # use the values below
         self.iparm[1] = 2          # nested dissection (METIS)
         self.iparm[34] = 1         # zero-based indices
+        for k, v in (iparm or {}).items():
+            self.iparm[k] = v
         self._call(12, np.zeros(1), np.zeros(1), 1)
 
     def _call(self, phase, b, x, nrhs):
@@ -107,9 +109,14 @@ def inertia(S: sp.spmatrix, threads: int | None = None) -> tuple[int, int]:
     With S = A - tau B (B positive definite) the negative count is the number of
     generalized eigenvalues below tau (Sylvester's law of inertia)."""
     U = sp.triu(sp.csr_matrix(S)).tocsr()
-    P = Pardiso(U, threads=threads, mtype=-2)
-    out = int(P.iparm[21]), int(P.iparm[22])
+    # pivots perturbed by at most 1e-13 |S| (iparm(10)), Bunch-Kaufman pivoting (iparm(21)) with
+    # weighted matching (iparm(13)); a count made with perturbed pivots is refused, since a
+    # perturbation can flip the sign of an eigenvalue of S smaller than it
+    P = Pardiso(U, threads=threads, mtype=-2, iparm={9: 13, 12: 1, 20: 1})
+    out, perturbed = (int(P.iparm[21]), int(P.iparm[22])), int(P.iparm[13])
     P.release()
+    if perturbed:
+        raise RuntimeError(f"inertia: {perturbed} perturbed pivots; the eigenvalue count is not exact")
     return out
 
 
