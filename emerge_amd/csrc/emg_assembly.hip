@@ -1108,6 +1108,7 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
     constexpr int SPR = CH / 8;             // tet records per round: 8 lanes per record, 7 of them load 6 doubles
     constexpr int RR = (TS + SPR - 1) / SPR;
     static_assert(7 * 6 >= TD_FIELDS && 7 * 6 <= REC, "7 lanes of 6 doubles cover a record");
+    static_assert(RR * SPR <= CHUNK_TMAX, "tet_ids reads slots below CHUNK_TMAX only");
     __shared__ uint32_t rcp[100 * RECIPE_BYTES / 4];
     __shared__ __align__(16) double td[TD > SD ? TD : SD];
     __shared__ int32_t s_pp[CH + 1];        // entry -> first incidence (absolute)
@@ -1121,22 +1122,34 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
     cplx *stg = reinterpret_cast<cplx *>(td);
     // registers holding the next chunk's data; every one is assigned on every
     // path, so none stays live through the incidence loop
-    int32_t f_q0 = 0, f_ninc = 0, f_nr = 0, f_nt = 0, g_nt = 0;   // uniform
+    int32_t f_q0 = 0, f_ninc = 0, f_nr = 0, f_nt = 0;              // uniform
+    int32_t g_nt = 0;                                               // uniform
+    int32_t g_u = 0;   // lanes 0-3 of each wave: the next chunk's first and end incidence, first and end row
     int32_t f_pp[2], f_row[2], g_t[RR];
     uint32_t f_inc[RI];
     int f_perm = 0;
     double2 f_rec[RR][3];
+    // The chunk's tet ids and its uniform offsets, issued before the incidence loop.
+    // Unconditional loads (the chunk index clamped, every slot below CHUNK_TMAX in
+    // bounds): nothing here depends on a loaded value, so nothing waits for these
+    // loads before prefetch uses them; slots at or past the tet count are never used.
     auto tet_ids = [&](int64_t cc) {
-        g_nt = cc < pat.nchunk ? pat.chunk_ntets[cc] : 0;
+        const int64_t cl = cc < pat.nchunk ? cc : pat.nchunk - 1;
+        const int64_t kl = cl * CH;
+        g_nt = pat.chunk_ntets[cl];
+        // one gather, lane by lane, so the values stay in vector registers until
+        // prefetch reads them (a uniform load would be moved to a scalar register,
+        // which waits for it at once)
+        const int32_t *src = (tid & 3) == 0   ? pat.pair_ptr + kl
+                             : (tid & 3) == 1 ? pat.pair_ptr + (pat.nnz_ent - kl < CH ? pat.nnz_ent : kl + CH)
+                             : (tid & 3) == 2 ? pat.chunk_row0 + cl
+                                              : pat.chunk_row0 + cl + 1;
+        g_u = *src;
 #pragma unroll
-        for (int h = 0; h < RR; h++) {
-            const int slot = h * SPR + rs;
-            g_t[h] = slot < g_nt ? pat.chunk_tets[cc * CHUNK_TMAX + slot] : 0;
-        }
+        for (int h = 0; h < RR; h++) g_t[h] = pat.chunk_tets[cl * CHUNK_TMAX + h * SPR + rs];
     };
     auto prefetch = [&](int64_t cc) {
-        f_q0 = 0; f_ninc = 0; f_nr = 0; f_perm = 0;
-        f_nt = g_nt;
+        f_q0 = 0; f_ninc = 0; f_nr = 0; f_perm = 0; f_nt = 0;
 #pragma unroll
         for (int h = 0; h < 2; h++) { f_pp[h] = 0; f_row[h] = 0; }
 #pragma unroll
@@ -1146,12 +1159,13 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
 #pragma unroll
             for (int q = 0; q < 3; q++) f_rec[h][q] = make_double2(0.0, 0.0);
         if (cc >= pat.nchunk) return;
+        f_nt = g_nt;
         const int64_t kk = cc * CH;
         const int npp = (int)(pat.nnz_ent - kk < CH ? pat.nnz_ent - kk : CH);
-        f_q0 = pat.pair_ptr[kk];
-        f_ninc = pat.pair_ptr[kk + npp] - f_q0;
-        const int32_t r0 = pat.chunk_row0[cc];
-        f_nr = pat.chunk_row0[cc + 1] - r0 + 2;
+        f_q0 = __builtin_amdgcn_readlane(g_u, 0);
+        f_ninc = __builtin_amdgcn_readlane(g_u, 1) - f_q0;
+        const int32_t r0 = __builtin_amdgcn_readlane(g_u, 2);
+        f_nr = __builtin_amdgcn_readlane(g_u, 3) - r0 + 2;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int i = tid + h * CH;
@@ -1174,12 +1188,12 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
                 for (int q = 0; q < 3; q++) f_rec[h][q] = r[q];
             }
     };
-    tet_ids(blockIdx.x);
-    prefetch(blockIdx.x);
-    for (int64_t c = blockIdx.x; c < pat.nchunk; c += gridDim.x) {
-        __syncthreads();   // the previous chunk's staged output has been written
-        const int64_t k0 = c * CH;
-        const int np = (int)(pat.nnz_ent - k0 < CH ? pat.nnz_ent - k0 : CH);
+    // the prefetched registers of chunk cc into LDS; the chunk's row count and first
+    // incidence index stay in cur_nr / cur_qb
+    int32_t cur_nr = 0, cur_qb = 0;
+    auto stage = [&](int64_t cc) {
+        const int64_t kk = cc * CH;
+        const int npp = cc < pat.nchunk ? (int)(pat.nnz_ent - kk < CH ? pat.nnz_ent - kk : CH) : 0;
         const int32_t ioff = f_q0 & 1, nr = f_nr, nT = f_nt;
 #pragma unroll
         for (int h = 0; h < RR; h++) {
@@ -1195,7 +1209,7 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int i = tid + h * CH;
-            if (i <= np) s_pp[i] = f_pp[h];
+            if (i <= npp) s_pp[i] = f_pp[h];
             if (i < nr) s_row[i] = f_row[h];
         }
         {
@@ -1207,8 +1221,17 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
             }
         }
         s_perm[tid] = (uint8_t)f_perm;
-        const int32_t qb = f_q0 - ioff;   // incidence index of s_inc's first 16-bit half
-        __syncthreads();
+        cur_nr = nr;
+        cur_qb = f_q0 - ioff;   // incidence index of s_inc's first 16-bit half
+    };
+    tet_ids(blockIdx.x);
+    prefetch(blockIdx.x);
+    stage(blockIdx.x);
+    for (int64_t c = blockIdx.x; c < pat.nchunk; c += gridDim.x) {
+        __syncthreads();   // the chunk's index data and Gram records are in LDS
+        const int64_t k0 = c * CH;
+        const int np = (int)(pat.nnz_ent - k0 < CH ? pat.nnz_ent - k0 : CH);
+        const int32_t nr = cur_nr, qb = cur_qb;
         tet_ids(c + gridDim.x);
         const int hw = (int)((c + NW / 2) % NW);   // wave hw takes the first (heaviest) 64 lane-order positions
         const int lk = s_perm[(((tid >> 6) - hw + NW) % NW) * 64 + (tid & 63)];
@@ -1237,12 +1260,13 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
             s_out[lk] = make_int2(p0, deg);
         }
         __syncthreads();
+        // the blocks in entry order into registers, through the stage
         int2 o = make_int2(0, 0);
+        cplx ev[4], bv[4];
         if (tid < np) {
             o = s_out[tid];
 #pragma unroll
-            for (int m = 0; m < 4; m++)
-                E[(int64_t)(m >> 1) * 2 * pat.nnz_ent + o.x + (int64_t)(m & 1) * o.y + k0 + tid] = stg[m * CH + tid];
+            for (int m = 0; m < 4; m++) ev[m] = stg[m * CH + tid];
         }
         __syncthreads();
         if (act) {
@@ -1252,8 +1276,19 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
         __syncthreads();
         if (tid < np) {
 #pragma unroll
-            for (int m = 0; m < 4; m++)
-                B[(int64_t)(m >> 1) * 2 * pat.nnz_ent + o.x + (int64_t)(m & 1) * o.y + k0 + tid] = stg[m * CH + tid];
+            for (int m = 0; m < 4; m++) bv[m] = stg[m * CH + tid];
+        }
+        __syncthreads();   // LDS free: the next chunk's data goes in before this chunk's stores
+        // are issued, so waiting for the prefetched loads does not wait for the stores
+        // (loads and stores share one counter); the stores drain during the next chunk
+        stage(c + gridDim.x);
+        if (tid < np) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int64_t a = (int64_t)(m >> 1) * 2 * pat.nnz_ent + o.x + (int64_t)(m & 1) * o.y + k0 + tid;
+                E[a] = ev[m];
+                B[a] = bv[m];
+            }
         }
     }
 }
