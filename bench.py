@@ -26,10 +26,10 @@ METRIC = "sweep freq-pts/s at ≤1e-8 FP64 residual, learned vs plain GPU; assem
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per numeric assembly (k_tet_gram + k_assemble_staged) on the
 # 1,005,000-tet workload, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
-# runs of this benchmark (--sweep none --copy-calibration): FETCH (52,925 +
-# 1,702,889) KiB x 2 + WRITE (425,980 + 8,794,723) KiB x 1, the factors a 2 GiB
+# runs of this benchmark (--sweep none --no-eigen --copy-calibration): FETCH (53,747 +
+# 1,301,548) KiB x 2 + WRITE (430,714 + 8,764,685) KiB x 1, the factors a 2 GiB
 # device-to-device copy in the same runs gives.
-ASSEMBLY_TRAFFIC_BYTES = ((52_925 + 1_702_889) * 2 + 425_980 + 8_794_723) * 1024
+ASSEMBLY_TRAFFIC_BYTES = ((53_747 + 1_301_548) * 2 + 430_714 + 8_764_685) * 1024
 # HBM bytes per launch of the sweep's complex64 SpMV (k_spmv_blk<float,16,false>) on the
 # posts box, from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this benchmark
 # (--spmv-kernel-only --copy-calibration): FETCH 602,184 KiB x 2 + WRITE 17,228 KiB x 1,

@@ -66,7 +66,7 @@ struct emg_pattern {
     int32_t *chunk_ntets = nullptr;  // nchunk
     int chunk_tmax = 0;              // largest chunk_ntets of this mesh (sizes the fused kernel's LDS)
     uint16_t *inc_local = nullptr;   // n_inc: (local tet << 8) | (s << 4) | s'
-    uint8_t *chunk_perm = nullptr;   // nchunk * CHUNK: chunk-local entries by descending incidence count
+    uint16_t *chunk_perm = nullptr;  // nchunk * CHUNK: chunk-local entries by descending incidence count
     double *tet_rec = nullptr;       // nt * REC: per-tet Gram records (fused-path workspace)
     double *tet_rec_batch = nullptr; // rec_batch_cap * nt * REC: the records of a batch of meshes
     int rec_batch_cap = 0;
@@ -81,15 +81,15 @@ static constexpr int N_UPPER = 55;
 static constexpr int TET_BLOCK_DOUBLES = 4 * (N_UPPER + N_UPPER + 6);
 // Fused path: a workgroup owns `chunk` (at most CHUNK) consecutive adjacency
 // entries and stages the Gram data of their (at most CHUNK_TMAX) distinct tets in LDS.
-static constexpr int CHUNK = 256;
+static constexpr int CHUNK = 512;
 static constexpr int CHUNK_TMAX = 128;
-static constexpr int CHUNK_NMAX = 1024;  // incidences per chunk handled by the tet-list builder
+static constexpr int CHUNK_NMAX = 2048;  // incidences per chunk handled by the tet-list builder
 static constexpr int RECIPE_BYTES = 48;
 
 static void build_recipes(uint8_t *out);
-__global__ void __launch_bounds__(256) k_chunk_tets(const emg_pattern pat, int32_t *__restrict__ chunk_tets,
-                                                    int32_t *__restrict__ chunk_ntets, uint16_t *__restrict__ inc_local,
-                                                    uint8_t *__restrict__ chunk_perm, int *__restrict__ overflow);
+__global__ void __launch_bounds__(CHUNK) k_chunk_tets(const emg_pattern pat, int32_t *__restrict__ chunk_tets,
+                                                      int32_t *__restrict__ chunk_ntets, uint16_t *__restrict__ inc_local,
+                                                      uint16_t *__restrict__ chunk_perm, int *__restrict__ overflow);
 
 // ------------------------------------------------------------------ symbolic
 __global__ void k_make_keys(const int32_t *__restrict__ tet_ent, int64_t nt, int bits,
@@ -264,14 +264,14 @@ extern "C" int emg_pattern_build(const int32_t *d_tet_ent, int64_t nt, int64_t n
         TRY(hipMalloc(&pat->chunk_ntets, (size_t)pat->nchunk * sizeof(int32_t)));
         // two spare codes: the fused kernel reads the codes as 32-bit words
         TRY(hipMalloc(&pat->inc_local, (size_t)(n_inc + 2) * sizeof(uint16_t)));
-        TRY(hipMalloc(&pat->chunk_perm, (size_t)pat->nchunk * pat->chunk));
+        TRY(hipMalloc(&pat->chunk_perm, (size_t)pat->nchunk * pat->chunk * sizeof(uint16_t)));
         TRY(hipMalloc(&pat->recipes, 100 * RECIPE_BYTES));
         TRY(hipMalloc(&d_over, sizeof(int)));
         TRY(hipMemsetAsync(d_over, 0, sizeof(int), st));
         uint8_t host_rec[100 * RECIPE_BYTES];
         build_recipes(host_rec);
         TRY(hipMemcpyAsync(pat->recipes, host_rec, sizeof(host_rec), hipMemcpyHostToDevice, st));
-        k_chunk_tets<<<(unsigned)pat->nchunk, 256, 0, st>>>(*pat, pat->chunk_tets, pat->chunk_ntets, pat->inc_local,
+        k_chunk_tets<<<(unsigned)pat->nchunk, CHUNK, 0, st>>>(*pat, pat->chunk_tets, pat->chunk_ntets, pat->inc_local,
                                                              pat->chunk_perm, d_over);
         TRY(hipGetLastError());
         int over = 0;
@@ -800,9 +800,9 @@ static void build_recipes(uint8_t *out) {
 }
 
 // Distinct tets of every chunk, and chunk-local incidence codes.
-__global__ void __launch_bounds__(256) k_chunk_tets(const emg_pattern pat, int32_t *__restrict__ chunk_tets,
-                                                    int32_t *__restrict__ chunk_ntets, uint16_t *__restrict__ inc_local,
-                                                    uint8_t *__restrict__ chunk_perm, int *__restrict__ overflow) {
+__global__ void __launch_bounds__(CHUNK) k_chunk_tets(const emg_pattern pat, int32_t *__restrict__ chunk_tets,
+                                                      int32_t *__restrict__ chunk_ntets, uint16_t *__restrict__ inc_local,
+                                                      uint16_t *__restrict__ chunk_perm, int *__restrict__ overflow) {
     __shared__ uint32_t key[CHUNK_NMAX];
     __shared__ uint32_t uq[CHUNK_NMAX];
     __shared__ uint8_t ckey[CHUNK];
@@ -830,19 +830,19 @@ __global__ void __launch_bounds__(256) k_chunk_tets(const emg_pattern pat, int32
             const int kj = ckey[j];
             rank += (kj < myk || (kj == myk && j < tid)) ? 1 : 0;
         }
-        chunk_perm[k0 + rank] = (uint8_t)tid;
+        chunk_perm[k0 + rank] = (uint16_t)tid;
     } else if (tid < ch) {
-        chunk_perm[k0 + tid] = (uint8_t)tid;
+        chunk_perm[k0 + tid] = (uint16_t)tid;
     }
     if (n > CHUNK_NMAX * ch / CHUNK) {   // the fused kernel stages at most this many incidences
         if (tid == 0) { atomicOr(overflow, 1); chunk_ntets[c] = -1; }
         return;
     }
-    for (int i = tid; i < CHUNK_NMAX; i += 256) key[i] = (i < n) ? (pat.inc[q0 + i] >> 8) : 0xFFFFFFFFu;
+    for (int i = tid; i < CHUNK_NMAX; i += CHUNK) key[i] = (i < n) ? (pat.inc[q0 + i] >> 8) : 0xFFFFFFFFu;
     __syncthreads();
     for (int len = 2; len <= CHUNK_NMAX; len <<= 1)
         for (int stride = len >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < CHUNK_NMAX; i += 256) {
+            for (int i = tid; i < CHUNK_NMAX; i += CHUNK) {
                 const int j = i ^ stride;
                 if (j > i) {
                     const bool asc = (i & len) == 0;
@@ -864,9 +864,9 @@ __global__ void __launch_bounds__(256) k_chunk_tets(const emg_pattern pat, int32
         if (tid == 0) { atomicOr(overflow, 1); chunk_ntets[c] = -1; }
         return;
     }
-    for (int i = tid; i < m; i += 256) chunk_tets[c * CHUNK_TMAX + i] = (int32_t)uq[i];
+    for (int i = tid; i < m; i += CHUNK) chunk_tets[c * CHUNK_TMAX + i] = (int32_t)uq[i];
     if (tid == 0) chunk_ntets[c] = m;
-    for (int i = tid; i < n; i += 256) {
+    for (int i = tid; i < n; i += CHUNK) {
         const uint32_t v = pat.inc[q0 + i];
         const uint32_t t = v >> 8;
         int lo = 0, hi = m - 1;
@@ -1114,7 +1114,7 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
     __shared__ int32_t s_pp[CH + 1];        // entry -> first incidence (absolute)
     __shared__ int32_t s_row[CH + 2];       // ent_ptr of the rows the chunk spans
     __shared__ uint32_t s_inc[NID];         // the chunk's incidence codes (16-bit), from an even index
-    __shared__ uint8_t s_perm[CH];          // lane order
+    __shared__ uint16_t s_perm[CH];         // lane order
     __shared__ int2 s_out[CH];              // per entry: (row start, row length)
     const int tid = threadIdx.x;
     const int rs = tid >> 3, rsub = tid & 7;   // record slot within a round, 6-double part of the record
@@ -1220,7 +1220,7 @@ __global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
                 if (i < nwords) s_inc[i] = f_inc[h];
             }
         }
-        s_perm[tid] = (uint8_t)f_perm;
+        s_perm[tid] = (uint16_t)f_perm;
         cur_nr = nr;
         cur_qb = f_q0 - ioff;   // incidence index of s_inc's first 16-bit half
     };
