@@ -87,12 +87,8 @@ def test_cavity_workload_modes_match_oracle_fixture(cuda):
     import os
     from emerge_amd import workloads
     gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "oracle_fixtures.npz"))
-    if "c4_lambda" not in gold.files:
-        pytest.skip("the oracle's cavity eigenvalues are not in oracle_fixtures.npz "
-                    "(tests/golden/write_oracle_fixtures.py c4 writes them)")
     lam_o = np.sort(gold["c4_lambda"].real)
-    if "c4_modes_below" in gold.files:
-        assert list(gold["c4_modes_below"]) == [0, 10]
+    assert list(gold["c4_modes_below"]) == [0, 10]
     phys = workloads.cavity()
     print(f"cavity: {phys.mesh.n_tets} tets", flush=True)
     data = phys.eigenmode(num_sols=10, target_frequency=workloads.CAVITY_TARGET_HZ)
