@@ -6,8 +6,9 @@ does with ARPACK (fem/physics/edm/emfreq3d.py:838-848:
 ``eigs(A, k, M=B, sigma, which='LM')``) and its port-mode path with eigsh
 (fem/solver.py:343-357).
 
-* OP v = (E - sigma B)^{-1} B v, every application is a GPU GMRES solve
-  (complex128, tight tolerance) with the shifted matrix set once;
+* OP v = (E - sigma B)^{-1} B v, every application is a GPU GMRES solve to the
+  FP64 relative residual inner_tol (default 1e-10; complex128, or complex64 cycles
+  in FP64 refinement) with the shifted matrix set once;
 * Arnoldi with classical Gram-Schmidt + one re-orthogonalisation, all vector
   work through the library's complex128 primitives;
 * FP64 Rayleigh-Ritz: the pencil is projected onto the Krylov basis
@@ -30,7 +31,7 @@ from .bc import C0
 
 
 class ShiftInvertArnoldi:
-    def __init__(self, problem, sigma: float, ncv: int = 40, inner_tol: float = 1e-12, restart: int = 150,
+    def __init__(self, problem, sigma: float, ncv: int = 40, inner_tol: float = 1e-10, restart: int = 150,
                  inner_maxiter: int = 200000, seed: int = 0, preconditioner: str = "hybrid", verbose: bool = False,
                  poly_degree: int = 1, precision: int = 64):
         import torch
