@@ -1091,7 +1091,7 @@ __global__ void __launch_bounds__(256) k_tet_gram(int64_t nt, const double *__re
 // The wavefront taking the heaviest entries rotates with the chunk, so no SIMD
 // always gets them.  Launch bound: 4 waves per SIMD (3 for the widest LDS layout).
 template <int TS, int CH>
-__global__ void __launch_bounds__(CH, TS <= 64 ? 4 : 3)
+__global__ void __launch_bounds__(CH, TS <= 96 ? 4 : 3)
     k_assemble_staged(const emg_pattern pat, const double *__restrict__ rec, int exact_face_mass,
                       cplx *__restrict__ E, cplx *__restrict__ B, int64_t rec_stride, int64_t out_stride) {
     // blockIdx.y: the mesh of a batch (its records and output values at these strides)
@@ -1303,6 +1303,8 @@ static void launch_staged(const emg_pattern *pat, dim3 grid, hipStream_t st, con
         k_assemble_staged<48, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB, rec_stride, out_stride);
     else if (tm <= 64)
         k_assemble_staged<64, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB, rec_stride, out_stride);
+    else if (tm <= 96)
+        k_assemble_staged<96, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB, rec_stride, out_stride);
     else
         k_assemble_staged<CHUNK_TMAX, CH><<<grid, CH, 0, st>>>(*pat, rec, efm, oE, oB, rec_stride, out_stride);
 }
