@@ -132,6 +132,16 @@ class GPUPattern:
                        "emg_assemble_tensor")
         return E, B
 
+    def assembly_tuning(self) -> dict:
+        """The fused assembly's autotuned grid (chosen on the pattern's first fused assembly):
+        workgroups per CU and the measured ms per assembly of each candidate (4, 8, 16, 32, 64);
+        wg_per_cu 0 before the first fused assembly."""
+        wg = ctypes.c_int()
+        ms = (ctypes.c_float * 5)()
+        _lib.check(_lib.load().emg_pattern_assembly_tuning(self._handle, ctypes.byref(wg), ms),
+                   "emg_pattern_assembly_tuning")
+        return {"wg_per_cu": int(wg.value), "ms": dict(zip((4, 8, 16, 32, 64), [float(v) for v in ms]))}
+
     def assemble_iso_device(self, eps, nu, flags, E, B):
         """Isotropic assembly from device-resident per-tet eps_r and 1/mu_r (complex128)."""
         L = _lib.load()

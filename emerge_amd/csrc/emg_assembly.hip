@@ -71,6 +71,8 @@ struct emg_pattern {
     double *tet_rec_batch = nullptr; // rec_batch_cap * nt * REC: the records of a batch of meshes
     int rec_batch_cap = 0;
     uint8_t *recipes = nullptr;      // 100 * RECIPE_BYTES
+    int wg_per_cu = 0;               // fused launch: workgroups per CU, 0 until tuned (first fused assembly)
+    float wg_ms[5] = {};             // measured ms per assembly for ASM_WG[i]
 };
 
 static constexpr int PAIR_BLOCK = 256;
@@ -1325,6 +1327,60 @@ static int assemble(const emg_pattern *pat, const double *d_nodes, const int32_t
     return 0;
 }
 
+// Autotuner of the fused launch's grid: workgroups per CU.  A grid-stride workgroup owns
+// whole chunks, so every grid gives the same values bit for bit; the grid only decides how
+// evenly the chunks' uneven costs spread over the resident workgroups.  The first fused
+// assembly on a pattern runs each candidate once untimed and twice between HIP events (so it
+// synchronises the stream once) and keeps the fastest for the pattern's life;
+// EMG_ASM_WG_PER_CU (one of the candidates) overrides the search.
+static const int ASM_WG[5] = {4, 8, 16, 32, 64};
+
+template <typename RUN>
+static int tune_assembly(emg_pattern *mp, RUN &run, hipStream_t st) {
+    if (const char *env = getenv("EMG_ASM_WG_PER_CU")) {
+        const int v = atoi(env);
+        mp->wg_per_cu = 32;
+        for (int w : ASM_WG)
+            if (v == w) mp->wg_per_cu = v;
+        return 0;
+    }
+    hipEvent_t ev[2];
+    EMG_HIP(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        emg::set_error("tune_assembly: hipEventCreate failed");
+        return -2;
+    }
+    hipError_t e = hipSuccess;
+    float best = 3.4e38f;
+    int bw = 32;
+    for (int i = 0; i < 5 && e == hipSuccess; i++) {
+        run(ASM_WG[i]);
+        e = hipEventRecord(ev[0], st);
+        for (int r = 0; r < 2 && e == hipSuccess; r++) run(ASM_WG[i]);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (e == hipSuccess) e = hipGetLastError();
+        ms /= 2.f;
+        mp->wg_ms[i] = ms;
+        if (e == hipSuccess && ms < best) { best = ms; bw = ASM_WG[i]; }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    EMG_HIP(e);
+    mp->wg_per_cu = bw;
+    return 0;
+}
+
+extern "C" int emg_pattern_assembly_tuning(const emg_pattern *pat, int *h_wg_per_cu, float *h_ms) {
+    EMG_CHECK(pat && h_wg_per_cu && h_ms, "emg_pattern_assembly_tuning: null argument");
+    *h_wg_per_cu = pat->wg_per_cu;
+    for (int i = 0; i < 5; i++) h_ms[i] = pat->wg_ms[i];
+    return 0;
+}
+
 extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, const int32_t *d_tets, const double *d_eps,
                                 const double *d_nu, int flags, double *d_E, double *d_B, void *stream) {
     EMG_CHECK(pat && d_nodes && d_tets && d_eps && d_nu && d_E && d_B, "emg_assemble_iso: null argument");
@@ -1336,20 +1392,23 @@ extern "C" int emg_assemble_iso(const emg_pattern *pat, const double *d_nodes, c
             EMG_HIP(hipGetDevice(&dev));
             EMG_HIP(hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev));
         }
-        // 4 resident workgroups per CU (register bound: 4 waves per SIMD); 8 rounds of them balance
-        // the chunks' uneven costs better than 2 (C2: 3.68 -> 3.55 ms; 1, 3, 12 and 16 rounds measured too)
-        const int64_t grid = std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * 32);
         const int efm = (flags & EMG_EXACT_FACE_MASS) ? 1 : 0;
         const cplx *ce = reinterpret_cast<const cplx *>(d_eps), *cn = reinterpret_cast<const cplx *>(d_nu);
         cplx *oE = reinterpret_cast<cplx *>(d_E), *oB = reinterpret_cast<cplx *>(d_B);
         if (!mp->tet_rec) EMG_HIP(hipMalloc(&mp->tet_rec, (size_t)pat->nt * REC * sizeof(double)));
-        if (flags & EMG_REFERENCE_GEOMETRY)
-            k_tet_gram<true><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec,
-                                                                         0, 0, 0);
-        else
-            k_tet_gram<false><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn, mp->tet_rec,
-                                                                          0, 0, 0);
-        launch_staged<CHUNK>(pat, dim3((unsigned)grid), st, mp->tet_rec, efm, oE, oB);
+        auto run = [&](int wg) {
+            if (flags & EMG_REFERENCE_GEOMETRY)
+                k_tet_gram<true><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn,
+                                                                             mp->tet_rec, 0, 0, 0);
+            else
+                k_tet_gram<false><<<emg::grid_for(pat->nt, 256), 256, 0, st>>>(pat->nt, d_nodes, d_tets, ce, cn,
+                                                                              mp->tet_rec, 0, 0, 0);
+            const int64_t grid = std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * wg);
+            launch_staged<CHUNK>(pat, dim3((unsigned)grid), st, mp->tet_rec, efm, oE, oB);
+        };
+        if (mp->wg_per_cu == 0)
+            if (int rc = tune_assembly(mp, run, st)) return rc;
+        run(mp->wg_per_cu);
         EMG_LAUNCH_CHECK();
         return 0;
     }
@@ -1393,10 +1452,11 @@ extern "C" int emg_assemble_iso_batched(const emg_pattern *pat, int nbatch, int6
         EMG_HIP(hipMalloc(&mp->tet_rec_batch, (size_t)nbatch * pat->nt * REC * sizeof(double)));
         mp->rec_batch_cap = nbatch;
     }
-    // every mesh gets the single-mesh grid (at most 32 workgroups per CU): the batch is the
+    // every mesh gets the single-mesh grid (the tuned workgroups per CU, else 32): the batch is the
     // single launches back to back without their gaps, and small meshes, whose own launch
     // cannot fill the GPU, fill it together
-    const int64_t gx = std::max<int64_t>(1, std::min<int64_t>(pat->nchunk, (int64_t)g_num_cus * 32));
+    const int64_t gx = std::max<int64_t>(1, std::min<int64_t>(pat->nchunk,
+                                                              (int64_t)g_num_cus * (pat->wg_per_cu ? pat->wg_per_cu : 32)));
     const int efm = (flags & EMG_EXACT_FACE_MASS) ? 1 : 0;
     const cplx *ce = reinterpret_cast<const cplx *>(d_eps), *cn = reinterpret_cast<const cplx *>(d_nu);
     cplx *oE = reinterpret_cast<cplx *>(d_E), *oB = reinterpret_cast<cplx *>(d_B);

@@ -50,6 +50,14 @@ int emg_pattern_csr(const emg_pattern *pat, int32_t *d_rowptr, int32_t *d_cols, 
 
 void emg_pattern_destroy(emg_pattern *pat);
 
+/* The pattern's autotuned fused-assembly grid: the first fused emg_assemble_iso on a
+ * pattern times 4, 8, 16, 32 and 64 workgroups per CU (HIP events, 2 runs each after one
+ * untimed, so that call synchronises its stream) and keeps the fastest; every grid gives the
+ * same values bit for bit.  h_wg_per_cu: the choice (0 before the first fused assembly);
+ * h_ms[5]: milliseconds per assembly in that order.  EMG_ASM_WG_PER_CU overrides the search.
+ * No reference counterpart (its element loop is a Numba prange).                       */
+int emg_pattern_assembly_tuning(const emg_pattern *pat, int *h_wg_per_cu, float *h_ms);
+
 /* Flags for emg_assemble_*. */
 #define EMG_EXACT_FACE_MASS 1 /* use the exact face-face mass integral instead of
                                  the reference's coefficient (fem/mth/tet.py:1036) */

@@ -169,3 +169,26 @@ def test_fused_equals_two_phase_on_posts_box(cuda):
         rel = float((a - b).abs().max() / b.abs().max())
         print("fused vs two-phase max rel", rel, flush=True)
         assert rel < 1e-13
+
+
+def test_assembly_grid_autotuner(cuda, monkeypatch):
+    """The fused launch's grid is tuned on a pattern's first assembly (4 to 64 workgroups per
+    CU): the choice is the fastest measured, and patterns forced to other grids
+    (EMG_ASM_WG_PER_CU) give the same values bit for bit, since each chunk belongs to one
+    workgroup whatever the grid."""
+    import torch
+    mesh = meshgen.waveguide_wr90()
+    er, ur = _materials(mesh.n_tets, np.random.default_rng(7))
+    pat = GPUPattern(mesh)
+    assert pat.assembly_tuning()["wg_per_cu"] == 0
+    E1, B1 = pat.assemble(er, ur)
+    t = pat.assembly_tuning()
+    print("assembly tuning", t, flush=True)
+    assert t["wg_per_cu"] in t["ms"] and all(v > 0 for v in t["ms"].values())
+    assert t["ms"][t["wg_per_cu"]] == min(t["ms"].values())
+    for wg in (4, 64):
+        monkeypatch.setenv("EMG_ASM_WG_PER_CU", str(wg))
+        p2 = GPUPattern(mesh)
+        E2, B2 = p2.assemble(er, ur)
+        assert p2.assembly_tuning()["wg_per_cu"] == wg
+        assert torch.equal(E1, E2) and torch.equal(B1, B2)
