@@ -46,6 +46,7 @@ SIGNATURES: dict[str, tuple] = {
     "emg_farfield": (ctypes.c_int, [I64, P, P, P, P, I64, P, P, F64, P, P, P]),
     "emg_locate_points": (ctypes.c_int, [I64, P, P, P, P, P, P, P, P, P]),
     "emg_solver_spmv_tuning": (ctypes.c_int, [P, P, P]),
+    "emg_solver_smoother_tuning": (ctypes.c_int, [P, P, P]),
     "emg_pattern_assembly_tuning": (ctypes.c_int, [P, P, P]),
     "emg_assemble_iso_batched": (ctypes.c_int, [P, ctypes.c_int, I64, P, P, P, P, ctypes.c_int, P, P, P]),
     "emg_topology_build": (ctypes.c_int, [I64, I64, P, P, P, P]),

@@ -476,30 +476,40 @@ __global__ void __launch_bounds__(RB) k_dsinv(int64_t np2, const double *__restr
     if (inv32) { inv32[2 * p] = (float)r; inv32[2 * p + 1] = (float)im; }
 }
 
-// t[p] = dsinv[p] * sum over row p of G^T of g * r  (16-lane group per row:
-// vertex rows of G^T hold tens of entries)
-template <typename R>
+// t[p] = dsinv[p] * sum over row p of G^T of g * r, GL lanes per row (vertex rows of G^T
+// hold tens of entries, edge rows about ten; the solver's tuner picks GL)
+template <typename R, int GL>
 __global__ void __launch_bounds__(256) k_gt_apply(int64_t np2, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                   const R *__restrict__ gv, const R *__restrict__ dsinv,
                                                   const R *__restrict__ r, R *__restrict__ t) {
-    const int64_t p = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / SUB;
-    const int lane = threadIdx.x % SUB;
+    const int64_t p = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / GL;
+    const int lane = threadIdx.x % GL;
     if (p >= np2) return;
     double ar = 0.0, ai = 0.0;
-    for (int32_t e = rp[p] + lane; e < rp[p + 1]; e += SUB) {
+    for (int32_t e = rp[p] + lane; e < rp[p + 1]; e += GL) {
         const cx<R> v = ld(r, ci[e]);
         ar += (double)gv[e] * v.re;
         ai += (double)gv[e] * v.im;
     }
 #pragma unroll
-    for (int off = SUB / 2; off > 0; off >>= 1) {
-        ar += __shfl_xor(ar, off, SUB);
-        ai += __shfl_xor(ai, off, SUB);
+    for (int off = GL / 2; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off, GL);
+        ai += __shfl_xor(ai, off, GL);
     }
     if (lane == 0) {
         const cx<R> d = ld(dsinv, p);
         st(t, p, cx<R>((R)(d.re * ar - d.im * ai), (R)(d.re * ai + d.im * ar)));
     }
+}
+
+template <typename R>
+static void launch_gt(hipStream_t st, int L, int64_t np2, const int32_t *rp, const int32_t *ci, const R *gv,
+                      const R *dsinv, const R *r, R *t) {
+    if (L != 4 && L != 16) L = 8;
+    const unsigned grid = (unsigned)((np2 * L + 255) / 256);
+    if (L == 4) k_gt_apply<R, 4><<<grid, 256, 0, st>>>(np2, rp, ci, gv, dsinv, r, t);
+    else if (L == 16) k_gt_apply<R, 16><<<grid, 256, 0, st>>>(np2, rp, ci, gv, dsinv, r, t);
+    else k_gt_apply<R, 8><<<grid, 256, 0, st>>>(np2, rp, ci, gv, dsinv, r, t);
 }
 
 // z = BJ(r) + G t, both modes of entity e per thread
@@ -678,6 +688,10 @@ struct emg_solver {
     // the first matrix (tune_spmv), with the measured milliseconds per variant
     int lanes[2] = {0, 0};
     float tune_ms[2][4] = {};
+    // lanes per row of the smoother's G^T product (k_gt_apply): 0 until tuned with the
+    // SpMV (gradient set), then 4, 8 or 16, with the measured milliseconds per launch
+    int gt_lanes = 0;
+    float gt_ms[3] = {};
     struct graphs {
         std::vector<hipGraphExec_t> exec;
         const void *vals = nullptr, *inv = nullptr, *mask = nullptr;
@@ -708,7 +722,7 @@ static void apply_prec(emg_solver *s, const R *inv, const R *x, R *y, hipStream_
         const R *gtv = sizeof(R) == 8 ? reinterpret_cast<const R *>(s->gt_vals) : reinterpret_cast<const R *>(s->gt_vals32);
         const R *dsi = sizeof(R) == 8 ? reinterpret_cast<const R *>(s->dsinv64) : reinterpret_cast<const R *>(s->dsinv32);
         R *t = reinterpret_cast<R *>(s->tg);
-        k_gt_apply<R><<<(unsigned)((s->np2 * SUB + 255) / 256), 256, 0, st>>>(s->np2, s->gt_rowptr, s->gt_cols, gtv, dsi, x, t);
+        launch_gt<R>(st, s->gt_lanes, s->np2, s->gt_rowptr, s->gt_cols, gtv, dsi, x, t);
         k_bj_grad_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, x, s->g_rowptr, s->g_cols, gv, t, y);
     } else {
         k_bj_apply<R><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, inv, x, y);
@@ -949,16 +963,78 @@ static int tune_spmv_one(emg_solver *s, const R *vals, int slot, hipStream_t st)
     return 0;
 }
 
+// The smoother's G^T product with 4, 8 and 16 lanes per row, timed like the SpMV (one
+// untimed and five timed launches, in the precision of the solver's steps, on zeroed
+// vectors); EMG_GT_LANES (4, 8 or 16) overrides.  Needs the gradient correction set.
+static const int GT_LANES[3] = {4, 8, 16};
+
+static int tune_gt(emg_solver *s, hipStream_t st) {
+    if (s->np2 <= 0) return 0;
+    if (const char *env = getenv("EMG_GT_LANES")) {
+        const int v = atoi(env);
+        s->gt_lanes = (v == 4 || v == 16) ? v : 8;
+        return 0;
+    }
+    EMG_HIP(hipMemsetAsync(s->z, 0, s->n * 16, st));
+    hipEvent_t ev[2];
+    EMG_HIP(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        emg::set_error("tune_gt: hipEventCreate failed");
+        return -2;
+    }
+    hipError_t e = hipSuccess;
+    float best = 3.4e38f;
+    int bl = 8;
+    for (int i = 0; i < 3 && e == hipSuccess; i++) {
+        const int L = GT_LANES[i];
+        auto run = [&]() {
+            if (s->precision == 32)
+                launch_gt<float>(st, L, s->np2, s->gt_rowptr, s->gt_cols, s->gt_vals32, s->dsinv32,
+                                 reinterpret_cast<const float *>(s->z), reinterpret_cast<float *>(s->tg));
+            else
+                launch_gt<double>(st, L, s->np2, s->gt_rowptr, s->gt_cols, s->gt_vals, s->dsinv64, s->z, s->tg);
+        };
+        run();
+        e = hipEventRecord(ev[0], st);
+        for (int r = 0; r < 5; r++) run();
+        if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (e == hipSuccess) e = hipGetLastError();
+        ms /= 5.f;
+        s->gt_ms[i] = ms;
+        if (e == hipSuccess && ms < best) { best = ms; bl = L; }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    EMG_HIP(e);
+    s->gt_lanes = bl;
+    return 0;
+}
+
 static int tune_spmv(emg_solver *s, hipStream_t st) {
     if (getenv("EMG_SPMV_BLK_LANES") != nullptr) {
         s->lanes[0] = s->lanes[1] = spmv_blk_lanes();
-        return 0;
+        return tune_gt(s, st);
     }
     EMG_HIP(hipMemsetAsync(s->z, 0, s->n * 16, st));
     int rc = tune_spmv_one<double>(s, s->vals, 0, st);
     if (rc) return rc;
-    if (s->precision == 32) return tune_spmv_one<float>(s, s->vals32, 1, st);
-    s->lanes[1] = s->lanes[0];
+    if (s->precision == 32) {
+        rc = tune_spmv_one<float>(s, s->vals32, 1, st);
+        if (rc) return rc;
+    } else {
+        s->lanes[1] = s->lanes[0];
+    }
+    return tune_gt(s, st);
+}
+
+extern "C" int emg_solver_smoother_tuning(emg_solver *s, int *h_lanes, float *h_ms) {
+    EMG_CHECK(s && h_lanes && h_ms, "emg_solver_smoother_tuning: null argument");
+    *h_lanes = s->gt_lanes;
+    for (int i = 0; i < 3; i++) h_ms[i] = s->gt_ms[i];
     return 0;
 }
 
@@ -1011,8 +1087,7 @@ extern "C" int emg_solver_smoother_parts(emg_solver *s, const double *d_r, doubl
     hipStream_t st = (hipStream_t)stream;
     k_bj_apply<double><<<emg::grid_for(s->n_ent, RB), RB, 0, st>>>(s->n_ent, s->inv64, d_r, d_u);
     if (s->np2 > 0 && d_t != nullptr)
-        k_gt_apply<double><<<(unsigned)((s->np2 * SUB + 255) / 256), 256, 0, st>>>(s->np2, s->gt_rowptr, s->gt_cols,
-                                                                                  s->gt_vals, s->dsinv64, d_r, d_t);
+        launch_gt<double>(st, s->gt_lanes, s->np2, s->gt_rowptr, s->gt_cols, s->gt_vals, s->dsinv64, d_r, d_t);
     EMG_LAUNCH_CHECK();
     return 0;
 }

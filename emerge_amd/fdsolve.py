@@ -291,6 +291,15 @@ class GPUSolver:
                 "ms_c128": dict(zip(cand, [float(v) for v in ms[0:4]])),
                 "ms_c64": dict(zip(cand, [float(v) for v in ms[4:8]]))}
 
+    def smoother_tuning(self) -> dict:
+        """The autotuned lanes per row of the smoother's G^T product (chosen with the SpMV on
+        the solver's first matrix when the gradient correction is set; 0 otherwise, 8 used) and
+        the measured ms per launch of each candidate (4, 8, 16)."""
+        lanes = ctypes.c_int()
+        ms = (ctypes.c_float * 3)()
+        _lib.check(self.L.emg_solver_smoother_tuning(self.h, ctypes.byref(lanes), ms), "emg_solver_smoother_tuning")
+        return {"gt_lanes": int(lanes.value), "ms": dict(zip((4, 8, 16), [float(v) for v in ms]))}
+
     def set_weights(self, we, wp):
         """Learned smoother weights (complex128 device tensors or None) for the next set_matrix."""
         self._w = (we, wp)

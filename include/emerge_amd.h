@@ -167,6 +167,14 @@ int emg_solver_apply(emg_solver *s, int precision, const void *d_x, void *d_y, v
  * (4 per precision, in the order 4, 8, 16, 32).  EMG_SPMV_BLK_LANES overrides the search.
  * No reference counterpart (PARDISO has no shape to tune).                              */
 int emg_solver_spmv_tuning(emg_solver *s, int *h_lanes, float *h_ms);
+
+/* The solver's autotuned smoother shape: with the gradient correction set, the first
+ * emg_solver_set_matrix also times the smoother's G^T product with 4, 8 and 16 lanes per
+ * row (HIP events, 5 launches each, in the precision of the solver's steps) and keeps the
+ * fastest.  h_lanes[1]: the choice (0 before tuning or without a gradient: 8 is used);
+ * h_ms[3]: milliseconds per launch for 4, 8, 16.  EMG_GT_LANES overrides the search.
+ * No reference counterpart (its solvers are direct).                                     */
+int emg_solver_smoother_tuning(emg_solver *s, int *h_lanes, float *h_ms);
 /* d_x: initial guess in, solution out (n complex128). Synchronises `stream`. */
 int emg_solver_solve(emg_solver *s, const double *d_b, double *d_x, double tol, int maxiter, int max_refine,
                      emg_solve_info *info, void *stream);

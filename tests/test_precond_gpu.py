@@ -167,3 +167,42 @@ def test_solver_rejects_csr_off_the_entity_pattern(cuda):
     rc = L.emg_solver_create(40, A.nnz, _lib.ptr(rp), _lib.ptr(ci), 20, 64, ctypes.byref(h))
     assert rc != 0
     assert b"entity pattern" in L.emg_last_error()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_smoother_autotuner_same_result(cuda, precision, monkeypatch):
+    """The smoother's G^T product shape (4, 8 or 16 lanes per row) is tuned with the SpMV on
+    the solver's first matrix: every candidate has a measured time and the kept one is the
+    fastest; solvers forced to each shape (EMG_GT_LANES) give the same smoother parts up to
+    the order of the row sums."""
+    import torch
+    from emerge_amd import _lib
+    from emerge_amd.fdsolve import GPUSolver
+    mesh, phys = _posts(10, 5, 2)
+    prob = phys.problem()
+    K = prob.system(3e9)
+    s = GPUSolver(prob, precision=precision)
+    assert s.smoother_tuning()["gt_lanes"] == 0
+    s.set_matrix(K)
+    t = s.smoother_tuning()
+    print("smoother tuning", t, flush=True)
+    assert all(v > 0 for v in t["ms"].values())
+    assert t["gt_lanes"] == min(t["ms"], key=t["ms"].get)
+    L = _lib.load()
+    g = torch.Generator(device=prob.dev)
+    g.manual_seed(11)
+    r = torch.view_as_complex(torch.randn((prob.N, 2), generator=g, dtype=torch.float64, device=prob.dev)).contiguous()
+    np2 = prob.gradient().np2
+    parts = []
+    for lanes in (4, 8, 16):
+        monkeypatch.setenv("EMG_GT_LANES", str(lanes))
+        s2 = GPUSolver(prob, precision=64)
+        s2.set_matrix(K)
+        assert s2.smoother_tuning()["gt_lanes"] == lanes
+        u = torch.empty_like(r)
+        tg = torch.empty(np2, dtype=torch.complex128, device=prob.dev)
+        _lib.check(L.emg_solver_smoother_parts(s2.h, _lib.ptr(r), _lib.ptr(u), _lib.ptr(tg), _lib.stream_ptr()),
+                   "emg_solver_smoother_parts")
+        parts.append(tg)
+    for tg in parts[1:]:
+        assert float((tg - parts[0]).abs().max() / parts[0].abs().max()) < 1e-13
