@@ -129,7 +129,9 @@ def bench_sweep(args, dev):
     import torch
     from emerge_amd.physics import Electrodynamics3D
     mesh, ports, (f0, f1), desc = sweep_workload(args.sweep)
-    freqs = list(np.linspace(f0, f1, args.sweep_freqs))
+    # the workload's own point count unless given: 101 over 2-3 GHz for the patch antenna, 201 otherwise
+    nf = args.sweep_freqs if args.sweep_freqs else (101 if args.sweep == "patch" else 201)
+    freqs = list(np.linspace(f0, f1, nf))
     out = {"workload": desc, "tets": mesh.n_tets, "dofs": 2 * mesh.n_entities, "freqs": len(freqs),
            "band_hz": [f0, f1], "tol": args.sweep_tol}
     runs = {}
@@ -539,7 +541,7 @@ def main():
     ap.add_argument("--two-phase", action="store_true", help="time the per-tet-block fallback kernels")
     ap.add_argument("--sweep", default="posts", choices=["none", "wr90", "posts", "patch"],
                     help="frequency-sweep workload reported beside the assembly figure")
-    ap.add_argument("--sweep-freqs", type=int, default=201)
+    ap.add_argument("--sweep-freqs", type=int, default=0, help="points of the sweep (0: 201, or 101 for patch)")
     ap.add_argument("--plain-sample", type=int, default=3,
                     help="frequencies solved by plain FP64 GMRES (band ends and middle first)")
     ap.add_argument("--sweep-tol", type=float, default=1e-8)
