@@ -807,7 +807,8 @@ __global__ void __launch_bounds__(CHUNK) k_chunk_tets(const emg_pattern pat, int
                                                       uint16_t *__restrict__ chunk_perm, int *__restrict__ overflow) {
     __shared__ uint32_t key[CHUNK_NMAX];
     __shared__ uint32_t uq[CHUNK_NMAX];
-    __shared__ uint8_t ckey[CHUNK];
+    __shared__ int hist[256], base[256];       // entries per key, and those with a smaller key
+    __shared__ int wcnt[CHUNK / 64][256];      // entries per key in each wavefront
     __shared__ int nuniq;
     const int64_t c = blockIdx.x;
     const int tid = threadIdx.x;
@@ -819,20 +820,42 @@ __global__ void __launch_bounds__(CHUNK) k_chunk_tets(const emg_pattern pat, int
     // descending incidence count, so a wavefront's lanes loop a similar number
     // of times.  Position r of the chunk holds local entry chunk_perm[k0 + r].
     const int np = (int)(k1 - k0);
+    const bool has = tid < np;
     int myk = 255;
-    if (tid < np) {
+    if (has) {
         const int cnt = pat.pair_ptr[k0 + tid + 1] - pat.pair_ptr[k0 + tid];
         myk = 255 - (cnt < 255 ? cnt : 255);
     }
-    ckey[tid] = (uint8_t)myk;
+    // Stable counting rank: the entries with a smaller key, the entries with my key in
+    // earlier wavefronts, and those earlier in my wavefront.  A wavefront walks only the
+    // few distinct keys it holds (one ballot each).
+    for (int i = tid; i < 256; i += CHUNK) hist[i] = 0;
+    for (int i = tid; i < (CHUNK / 64) * 256; i += CHUNK) (&wcnt[0][0])[i] = 0;
     __syncthreads();
-    if (tid < np) {
-        int rank = 0;
-        for (int j = 0; j < np; j++) {
-            const int kj = ckey[j];
-            rank += (kj < myk || (kj == myk && j < tid)) ? 1 : 0;
+    const int wave = tid >> 6, lane = tid & 63;
+    int before = 0;
+    unsigned long long rest = __ballot(has);
+    while (rest) {
+        const int leader = __ffsll(rest) - 1;
+        const int k = __shfl(myk, leader);
+        const unsigned long long m = __ballot(has && myk == k);
+        if (has && myk == k) before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == leader) {
+            wcnt[wave][k] = __popcll(m);
+            atomicAdd(&hist[k], __popcll(m));
         }
-        chunk_perm[k0 + rank] = (uint16_t)tid;
+        rest &= ~m;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int i = 0; i < 256; i++) { base[i] = acc; acc += hist[i]; }
+    }
+    __syncthreads();
+    if (has) {
+        int rank = base[myk] + before;
+        for (int w = 0; w < wave; w++) rank += wcnt[w][myk];
+        if (rank < ch) chunk_perm[k0 + rank] = (uint16_t)tid;
     } else if (tid < ch) {
         chunk_perm[k0 + tid] = (uint16_t)tid;
     }
