@@ -21,8 +21,6 @@ does with ARPACK (fem/physics/edm/emfreq3d.py:838-848:
 """
 from __future__ import annotations
 
-import ctypes
-
 import numpy as np
 import scipy.linalg as sla
 

@@ -29,8 +29,6 @@ vector are sampled once and combined with the coefficients y_j.
 """
 from __future__ import annotations
 
-import time
-
 import numpy as np
 import scipy.sparse as sp
 
