@@ -25,7 +25,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from .assembly import GPUPattern
-from .bc import C0, PortBC
+from .bc import PortBC
 from .numbering import CallerNumbering, _caller_mesh
 
 _CONTEXTS: dict[int, tuple] = {}

@@ -29,12 +29,11 @@ that does not help is simply not used (the plain smoother is the fallback).
 """
 from __future__ import annotations
 
-import ctypes
 
 import numpy as np
 
 from . import _lib
-from .bc import C0, PEC, PortBC, RobinBC
+from .bc import C0, PEC, RobinBC
 
 FEATURES = 8
 PROJ = 16

@@ -12,7 +12,6 @@ data only (kind, tags, frame, dimensions), never their formulas.
 """
 from __future__ import annotations
 
-import ctypes
 
 import numpy as np
 import scipy.sparse as sp

@@ -10,7 +10,6 @@ bench.py as the host-core reference solve.
 from __future__ import annotations
 
 import ctypes
-import os
 
 import numpy as np
 import scipy.sparse as sp

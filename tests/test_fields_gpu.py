@@ -5,7 +5,6 @@ import numpy as np
 import pytest
 
 from emerge_amd import meshgen
-from emerge_amd.bc import C0
 from emerge_amd.fields import PointLocator
 from oracle import fd
 from oracle import oracle as O

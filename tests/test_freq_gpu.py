@@ -3,8 +3,7 @@ import numpy as np
 import pytest
 
 from emerge_amd import meshgen
-from emerge_amd.bc import (PEC, AbsorbingBoundary, CoordinateSystem, LumpedPort, Material,
-                           RectangularWaveguide)
+from emerge_amd.bc import AbsorbingBoundary, CoordinateSystem, LumpedPort, Material, RectangularWaveguide
 from emerge_amd.physics import Electrodynamics3D
 from oracle import fd
 from oracle import oracle as O

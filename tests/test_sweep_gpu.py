@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from emerge_amd import _lib, meshgen
+from emerge_amd import meshgen
 from emerge_amd.bc import C0, AbsorbingBoundary, CoordinateSystem, LumpedPort, Material
 from emerge_amd.physics import Electrodynamics3D
 from oracle import fd
